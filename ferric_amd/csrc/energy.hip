@@ -1,0 +1,487 @@
+// Fused RI-MP2 pair-energy kernels for gfx950 (CDNA4 MFMA).
+//
+// Replaces the per-i tail GEMM G_i = B_i^T B_tail plus the host pair loop of
+// the reference (crates/ferric-mp2/src/rimp2.rs:1399-1544; device twin
+// crates/ferric-mp2/src/rimp2_gpu.rs:198-271) with one launch in which
+// K_ij = B_i^T B_j never leaves the chip.
+//
+// Work item = (unique occupied pair i <= j, tile pair tA <= tB over the
+// virtuals). A workgroup (4 waves, 2x2) computes, over the whole Q range,
+//   C1(a, b) = sum_Q B_i[Q, a] B_j[Q, b] = (ia|jb)       a in A, b in B
+//   C2(a, b) = sum_Q B_j[Q, a] B_i[Q, b] = (ib|ja)
+// so g_ab and g_ba of every (a, b) sit in the same lane and register. The
+// pair loop's terms for (a, b) and for the mirrored (b, a) are
+//   OS: (g_ab^2 + g_ba^2)/D        SS: (g_ab - g_ba)^2/D
+// (the reference's g_ab^2/D and g_ab(g_ab - g_ba)/D summed over both
+// orientations; D is symmetric in a, b). A diagonal tile (tA == tB) covers
+// each unordered {a, b} twice and takes weight 1/2; the occupied pair takes
+// the reference's fac = 1 (i == j) or 2 (i < j). Every term is FP64 with one
+// divide; lane sums are compensated and every reduction runs in a fixed
+// order, so the result does not depend on scheduling.
+//
+// Operands come from the packed K-major layout of pack.hip:
+// row (i*nvir_pad + a) holds B^Q_ia for Q = 0..naux_pad-1.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "fh_device.h"
+#include "fh_kernels.h"
+
+namespace fh {
+
+namespace {
+
+template <typename T>
+struct Mfma4;
+template <>
+struct Mfma4<double> {
+    using acc = v4d;
+    __device__ __forceinline__ static acc mma(double x, double y, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma4<float> {
+    using acc = v4f;
+    __device__ __forceinline__ static acc mma(float x, float y, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+    }
+};
+
+struct Item {
+    int i, j, ta, tb;
+    int64_t id;
+};
+
+__device__ __forceinline__ Item decode_item(const EnergyArgs &a) {
+    Item it;
+    it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
+    const int64_t pair = it.id / a.ntp;
+    const int tp = (int)(it.id % a.ntp);
+    decode_pair(pair, a.nocc, it.i, it.j);
+    decode_tile_pair(tp, a.ntile, it.ta, it.tb);
+    return it;
+}
+
+// Weighted, compensated workgroup sum of (os, ss) -> 4 doubles at out4.
+__device__ __forceinline__ void block_store(KSum os, KSum ss, double w, double *out4) {
+    os.s *= w;
+    os.c *= w;
+    ss.s *= w;
+    ss.c *= w;
+    wave_merge(os);
+    wave_merge(ss);
+    __shared__ double red[16][4];
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane == 0) {
+        red[wid][0] = os.s;
+        red[wid][1] = os.c;
+        red[wid][2] = ss.s;
+        red[wid][3] = ss.c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KSum o, s;
+        const int nw = blockDim.x / kWave;
+        for (int k = 0; k < nw; ++k) {
+            o.merge(red[k][0], red[k][1]);
+            s.merge(red[k][2], red[k][3]);
+        }
+        out4[0] = o.s;
+        out4[1] = o.c;
+        out4[2] = s.s;
+        out4[3] = s.c;
+    }
+}
+
+// The FP64 epilogue of one (a, b): both orientations of the pair loop.
+__device__ __forceinline__ void pair_terms(double g1, double g2, double d, bool kap, double kappa,
+                                           KSum &os, KSum &ss) {
+    const double inv = 1.0 / d;
+    const double dd = g1 - g2;
+    double t_os = (g1 * g1 + g2 * g2) * inv;
+    double t_ss = (dd * dd) * inv;
+    if (kap) {
+        const double d1 = 1.0 - exp(kappa * d);
+        const double damp = d1 * d1;
+        t_os *= damp;
+        t_ss *= damp;
+    }
+    os.add(t_os);
+    ss.add(t_ss);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32: v_mfma_{f64,f32}_16x16x4. Lane (r = l%16, g = l/16) supplies
+// A[row r][k] and B[k][col r] for its k values; the output element v of lane
+// l is C[out_row(g, v)][r]. Within a panel of KP Q values lane group g owns the
+// contiguous k range [g*KPL, g*KPL + KPL), read with one 16-byte LDS load;
+// A and B use the same k assignment, so each MFMA contracts matching Q.
+// ---------------------------------------------------------------------------
+template <typename T, int TILE, int KP>
+__global__ __launch_bounds__(256) void energy_k4_kernel(EnergyArgs a) {
+    using M = Mfma4<T>;
+    constexpr int TW = TILE / 2;          // wave sub-tile side (2x2 waves)
+    constexpr int NB = TW / 16;           // 16x16 blocks per wave side
+    constexpr int EPC = 16 / sizeof(T);   // elements per 16-byte chunk
+    constexpr int CPR = KP / EPC;         // chunks per row per panel
+    constexpr int OPCH = TILE * CPR;      // chunks per operand panel
+    constexpr int NCH = 4 * OPCH;         // chunks per panel (4 operands)
+    constexpr int CPT = (NCH + 255) / 256;
+    constexpr int LDR = KP + EPC;         // LDS row stride: one chunk of padding
+    constexpr int KPL = KP / 4;           // k values per lane per panel
+    static_assert(KPL * sizeof(T) == 16, "one 16-byte fragment load per operand block");
+    typedef T frag_t __attribute__((ext_vector_type(KPL)));
+
+    __shared__ __attribute__((aligned(16))) T lds[2][4][TILE][LDR];
+
+    const Item it = decode_item(a);
+    const T *__restrict__ bp = (const T *)a.op0;
+    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
+    // Operand rows: 0 = (i, A), 1 = (j, B), 2 = (j, A), 3 = (i, B).
+    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
+    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
+    const int64_t base2 = (int64_t)it.j * a.nvir_pad + A0;
+    const int64_t base3 = (int64_t)it.i * a.nvir_pad + B0;
+    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
+    const int64_t ld = a.naux_pad;
+    const int tid = threadIdx.x;
+
+    v4i stg[CPT];
+    auto gload = [&](int q0) {
+#pragma unroll
+        for (int t = 0; t < CPT; ++t) {
+            const int c = tid + 256 * t;
+            const int op = c / OPCH, rem = c % OPCH;
+            const int row = rem / CPR, ch = rem % CPR;
+            const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
+            const int lim = (op & 1) ? limB : limA;
+            v4i v = {0, 0, 0, 0};
+            if (c < NCH && row < lim)
+                v = *(const v4i *)(bp + (base + row) * ld + q0 + ch * EPC);
+            stg[t] = v;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < CPT; ++t) {
+            const int c = tid + 256 * t;
+            if (c < NCH) {
+                const int op = c / OPCH, rem = c % OPCH;
+                const int row = rem / CPR, ch = rem % CPR;
+                *(v4i *)&lds[buf][op][row][ch * EPC] = stg[t];
+            }
+        }
+    };
+
+    const int lane = tid % kWave, wid = tid / kWave;
+    const int wr = wid / 2, wc = wid % 2;
+    const int r = lane % 16, g = lane / 16;
+    bool rv[NB], cv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        rv[k] = A0 + wr * TW + k * 16 < a.nvir_pad;
+        cv[k] = B0 + wc * TW + k * 16 < a.nvir_pad;
+    }
+
+    typename M::acc acc1[NB][NB], acc2[NB][NB];
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            acc1[x][y] = (typename M::acc){0, 0, 0, 0};
+            acc2[x][y] = (typename M::acc){0, 0, 0, 0};
+        }
+
+    auto compute = [&](int buf) {
+        frag_t f0[NB], f1[NB], f2[NB], f3[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            f0[k] = *(const frag_t *)&lds[buf][0][wr * TW + k * 16 + r][g * KPL];
+            f2[k] = *(const frag_t *)&lds[buf][2][wr * TW + k * 16 + r][g * KPL];
+            f1[k] = *(const frag_t *)&lds[buf][1][wc * TW + k * 16 + r][g * KPL];
+            f3[k] = *(const frag_t *)&lds[buf][3][wc * TW + k * 16 + r][g * KPL];
+        }
+#pragma unroll
+        for (int s = 0; s < KPL; ++s)
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+                    if (rv[x] && cv[y]) {
+                        acc1[x][y] = M::mma(f0[x][s], f1[y][s], acc1[x][y]);
+                        acc2[x][y] = M::mma(f2[x][s], f3[y][s], acc2[x][y]);
+                    }
+    };
+
+    const int np = (int)(a.naux_pad / KP);
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int p = 0; p < np; ++p) {
+        const int cur = p & 1;
+        if (p + 1 < np) gload((p + 1) * KP);
+        compute(cur);
+        if (p + 1 < np) lstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    // FP64 epilogue.
+    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
+    const bool kap = !isnan(a.kappa);
+    KSum os, ss;
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            if (!(rv[x] && cv[y])) continue;
+            const int bb = B0 + wc * TW + y * 16 + r;
+            if (bb >= a.nvir) continue;
+            const double eb = a.eps_vir[bb];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = A0 + wr * TW + x * 16 + out_row<T>(g, v);
+                if (aa < a.nvir) {
+                    const double d = eij - a.eps_vir[aa] - eb;
+                    pair_terms((double)acc1[x][y][v], (double)acc2[x][y][v], d, kap, a.kappa, os,
+                               ss);
+                }
+            }
+        }
+    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
+    block_store(os, ss, w, a.item_partials + 4 * it.id);
+}
+
+// ---------------------------------------------------------------------------
+// BF16 hi/lo three-pass: v_mfma_f32_16x16x32_bf16. Lane (r, g) supplies 8
+// consecutive k values [8g, 8g+8) of a 32-deep panel for A rows and B columns
+// alike. Each 128-byte LDS row holds the panel's 32 hi then 32 lo values as
+// eight 16-byte chunks, chunk c stored at slot c ^ (row & 7) so that the
+// fragment reads of 8 consecutive rows hit distinct banks.
+// C += hi*hi + hi*lo + lo*hi in FP32; the omitted lo*lo term is added back
+// exactly (FP64 row sums from pack.hip) on the diagonal elements K_ii[a, a],
+// the only ones where it is one-signed.
+// ---------------------------------------------------------------------------
+template <int TILE>
+__global__ __launch_bounds__(256) void energy_bf16x3_kernel(EnergyArgs a) {
+    constexpr int KP = 32;
+    constexpr int TW = TILE / 2;
+    constexpr int NB = TW / 16;
+    constexpr int CPR = 8;               // 4 hi + 4 lo chunks per row
+    constexpr int OPCH = TILE * CPR;
+    constexpr int NCH = 4 * OPCH;
+    constexpr int CPT = (NCH + 255) / 256;
+
+    __shared__ __attribute__((aligned(16))) v4i lds[2][4][TILE][CPR];
+
+    const Item it = decode_item(a);
+    const uint16_t *__restrict__ hi = (const uint16_t *)a.op0;
+    const uint16_t *__restrict__ lo = (const uint16_t *)a.op1;
+    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
+    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
+    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
+    const int64_t base2 = (int64_t)it.j * a.nvir_pad + A0;
+    const int64_t base3 = (int64_t)it.i * a.nvir_pad + B0;
+    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
+    const int64_t ld = a.naux_pad;
+    const int tid = threadIdx.x;
+
+    v4i stg[CPT];
+    auto gload = [&](int q0) {
+#pragma unroll
+        for (int t = 0; t < CPT; ++t) {
+            const int c = tid + 256 * t;
+            const int op = c / OPCH, rem = c % OPCH;
+            const int row = rem / CPR, ch = rem % CPR;
+            const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
+            const int lim = (op & 1) ? limB : limA;
+            v4i v = {0, 0, 0, 0};
+            if (c < NCH && row < lim) {
+                const uint16_t *src = (ch < 4 ? hi : lo) + (base + row) * ld + q0 + (ch & 3) * 8;
+                v = *(const v4i *)src;
+            }
+            stg[t] = v;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < CPT; ++t) {
+            const int c = tid + 256 * t;
+            if (c < NCH) {
+                const int op = c / OPCH, rem = c % OPCH;
+                const int row = rem / CPR, ch = rem % CPR;
+                lds[buf][op][row][ch ^ (row & 7)] = stg[t];
+            }
+        }
+    };
+
+    const int lane = tid % kWave, wid = tid / kWave;
+    const int wr = wid / 2, wc = wid % 2;
+    const int r = lane % 16, g = lane / 16;
+    bool rv[NB], cv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        rv[k] = A0 + wr * TW + k * 16 < a.nvir_pad;
+        cv[k] = B0 + wc * TW + k * 16 < a.nvir_pad;
+    }
+    v4f acc1[NB][NB], acc2[NB][NB];
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            acc1[x][y] = (v4f){0, 0, 0, 0};
+            acc2[x][y] = (v4f){0, 0, 0, 0};
+        }
+
+    auto frag = [&](int buf, int op, int row, int chunk) -> v8bf {
+        return __builtin_bit_cast(v8bf, lds[buf][op][row][chunk ^ (row & 7)]);
+    };
+    auto compute = [&](int buf) {
+        v8bf h0[NB], l0[NB], h2[NB], l2[NB], h1[NB], l1[NB], h3[NB], l3[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int ra = wr * TW + k * 16 + r, rb = wc * TW + k * 16 + r;
+            h0[k] = frag(buf, 0, ra, g);
+            l0[k] = frag(buf, 0, ra, 4 + g);
+            h2[k] = frag(buf, 2, ra, g);
+            l2[k] = frag(buf, 2, ra, 4 + g);
+            h1[k] = frag(buf, 1, rb, g);
+            l1[k] = frag(buf, 1, rb, 4 + g);
+            h3[k] = frag(buf, 3, rb, g);
+            l3[k] = frag(buf, 3, rb, 4 + g);
+        }
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y)
+                if (rv[x] && cv[y]) {
+                    v4f c1 = acc1[x][y], c2 = acc2[x][y];
+                    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0[x], h1[y], c1, 0, 0, 0);
+                    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l2[x], h3[y], c2, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0[x], l1[y], c1, 0, 0, 0);
+                    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h2[x], l3[y], c2, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0[x], h1[y], c1, 0, 0, 0);
+                    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h2[x], h3[y], c2, 0, 0, 0);
+                    acc1[x][y] = c1;
+                    acc2[x][y] = c2;
+                }
+    };
+
+    const int np = (int)(a.naux_pad / KP);
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int p = 0; p < np; ++p) {
+        const int cur = p & 1;
+        if (p + 1 < np) gload((p + 1) * KP);
+        compute(cur);
+        if (p + 1 < np) lstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
+    const bool kap = !isnan(a.kappa);
+    KSum os, ss;
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            if (!(rv[x] && cv[y])) continue;
+            const int bb = B0 + wc * TW + y * 16 + r;
+            if (bb >= a.nvir) continue;
+            const double eb = a.eps_vir[bb];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = A0 + wr * TW + x * 16 + out_row<float>(g, v);
+                if (aa < a.nvir) {
+                    const double d = eij - a.eps_vir[aa] - eb;
+                    double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
+                    if (it.i == it.j && aa == bb) {
+                        const double c = a.diag_corr[(int64_t)it.i * a.nvir_pad + aa];
+                        g1 += c;
+                        g2 += c;
+                    }
+                    pair_terms(g1, g2, d, kap, a.kappa, os, ss);
+                }
+            }
+        }
+    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
+    block_store(os, ss, w, a.item_partials + 4 * it.id);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions: per occupied row i (items of row i are contiguous:
+// pairs (i, j >= i) start at p0 = i*nocc - i(i-1)/2), then an ascending fold
+// over i (the reference's i-ordered serial sum, rimp2.rs:1533-1538).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const double *__restrict__ ip, int nocc,
+                                                          int ntp, double *__restrict__ rows) {
+    const int i = blockIdx.x;
+    const int64_t p0 = (int64_t)i * nocc - (int64_t)i * (i - 1) / 2;
+    const int64_t w0 = p0 * ntp, w1 = (p0 + (nocc - i)) * ntp;
+    KSum os, ss;
+    for (int64_t w = w0 + threadIdx.x; w < w1; w += 256) {
+        os.merge(ip[4 * w], ip[4 * w + 1]);
+        ss.merge(ip[4 * w + 2], ip[4 * w + 3]);
+    }
+    block_store(os, ss, 1.0, rows + 4 * i);
+}
+
+__global__ void fold_rows_kernel(const double *__restrict__ rows, int nocc,
+                                 double *__restrict__ result) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    KSum os, ss;
+    for (int i = 0; i < nocc; ++i) {
+        os.merge(rows[4 * i], rows[4 * i + 1]);
+        ss.merge(rows[4 * i + 2], rows[4 * i + 3]);
+        result[2 + 2 * i] = rows[4 * i] + rows[4 * i + 1];
+        result[3 + 2 * i] = rows[4 * i + 2] + rows[4 * i + 3];
+    }
+    result[0] = os.value();
+    result[1] = ss.value();
+}
+
+template <typename K>
+hipError_t launch_items(K kern, const EnergyArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.nitems), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
+    if (a.nitems <= 0 || a.nitems > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipError_t e = hipErrorInvalidValue;
+    switch (a.prec) {
+    case Prec::F64:
+        if (a.tile == 128) e = launch_items(energy_k4_kernel<double, 128, 8>, a, st);
+        else if (a.tile == 64) e = launch_items(energy_k4_kernel<double, 64, 8>, a, st);
+        else if (a.tile == 32) e = launch_items(energy_k4_kernel<double, 32, 8>, a, st);
+        break;
+    case Prec::F32:
+        if (a.tile == 128) e = launch_items(energy_k4_kernel<float, 128, 16>, a, st);
+        else if (a.tile == 64) e = launch_items(energy_k4_kernel<float, 64, 16>, a, st);
+        else if (a.tile == 32) e = launch_items(energy_k4_kernel<float, 32, 16>, a, st);
+        break;
+    case Prec::BF16X3:
+        if (a.tile == 128) e = launch_items(energy_bf16x3_kernel<128>, a, st);
+        else if (a.tile == 64) e = launch_items(energy_bf16x3_kernel<64>, a, st);
+        else if (a.tile == 32) e = launch_items(energy_bf16x3_kernel<32>, a, st);
+        break;
+    case Prec::FP8:
+        e = launch_energy_fp8(a, st);
+        break;
+    }
+    return e;
+}
+
+hipError_t launch_energy_reduce(const EnergyArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3((unsigned)a.nocc), dim3(256), 0, st,
+                       a.item_partials, a.nocc, a.ntp, a.row_partials);
+    hipLaunchKernelGGL(fold_rows_kernel, dim3(1), dim3(64), 0, st, a.row_partials, a.nocc,
+                       a.result);
+    return hipGetLastError();
+}
+
+}  // namespace fh

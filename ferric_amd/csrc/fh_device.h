@@ -1,0 +1,97 @@
+// Device-side helpers shared by the RI-MP2 kernels (gfx950 / CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned short v8u16 __attribute__((ext_vector_type(8)));
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, one L2 each
+
+// Row, inside a 16x16 result block, of accumulator element v of lane 16 g + r
+// (whose column is r), per MFMA instruction, as measured on gfx950:
+// v_mfma_f64_16x16x4f64 holds rows 4v + g; the FP32-accumulating 16x16
+// instructions (f32 x4, bf16 x32, scaled f8 x128) hold rows 4g + v.
+template <typename T>
+__device__ __forceinline__ constexpr int out_row(int g, int v) {
+    return 4 * g + v;
+}
+template <>
+__device__ __forceinline__ constexpr int out_row<double>(int g, int v) {
+    return 4 * v + g;
+}
+
+// Compensated (Neumaier) FP64 sum: value() = s + c.
+struct KSum {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x) {
+        double t = s + x;
+        if (fabs(s) >= fabs(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    // Merge another compensated sum: TwoSum of the heads, compensations added.
+    __device__ __forceinline__ void merge(double os, double oc) {
+        double t = s + os;
+        double bp = t - s;
+        double err = (s - (t - bp)) + (os - bp);
+        s = t;
+        c += oc + err;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+// Workgroups are dispatched round-robin over the XCDs (block b runs on XCD
+// b % 8). Renumber so that each XCD receives a contiguous range of work
+// items: consecutive items share operand panels, which then meet in the same
+// L2. Bijective on [0, n).
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
+    int64_t q = n / kXcds, r = n % kXcds;
+    int64_t x = b % kXcds, local = b / kXcds;
+    return x * q + (x < r ? x : r) + local;
+}
+
+// Unique occupied pair p (i ascending, then j from i) -> (i, j).
+__device__ __forceinline__ void decode_pair(int64_t p, int nocc, int &i, int &j) {
+    int ii = 0;
+    int64_t rem = p;
+    while (rem >= nocc - ii) {
+        rem -= nocc - ii;
+        ++ii;
+    }
+    i = ii;
+    j = ii + (int)rem;
+}
+
+// Tile pair t over nt tiles with tA <= tB (row-major upper triangle).
+__device__ __forceinline__ void decode_tile_pair(int t, int nt, int &ta, int &tb) {
+    int a = 0, rem = t;
+    while (rem >= nt - a) {
+        rem -= nt - a;
+        ++a;
+    }
+    ta = a;
+    tb = a + rem;
+}
+
+// Wave-level butterfly reduction of a compensated pair (deterministic order).
+__device__ __forceinline__ void wave_merge(KSum &k) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double os = __shfl_xor(k.s, off, kWave);
+        double oc = __shfl_xor(k.c, off, kWave);
+        k.merge(os, oc);
+    }
+}
+
+}  // namespace fh

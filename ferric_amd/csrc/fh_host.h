@@ -1,0 +1,100 @@
+// Host-side state and helpers shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/ferric_hip.h"
+#include "fh_kernels.h"
+
+namespace fh {
+
+enum SlotId {
+    kSlotStage = 0,  // FP64 staging copy of a host b_ov
+    kSlotOp0,        // packed operand (or BF16 hi)
+    kSlotOp1,        // BF16 lo
+    kSlotScale,      // FP8 scales
+    kSlotEps,        // eps_occ | eps_vir
+    kSlotItems,      // per-item partial sums
+    kSlotRows,       // per-row partial sums
+    kSlotResult,     // energies + row values
+    kSlotScalar,     // small scalars (absmax)
+    kSlotGamma,      // adjoint: FP64 gamma staging for host calls
+    kSlotT,          // adjoint: per-row T~ block
+    kSlotDeps,       // adjoint: dE/deps partials
+    kSlotBatch,      // batched: descriptors and work list
+    kSlotBatchOut,   // batched: per-domain partials
+    kSlotDiag,       // BF16: lo^2 row sums
+    kNumSlots
+};
+
+struct Slot {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Pending {
+    uint64_t h2d_bytes = 0, d2h_bytes = 0, kernel_launches = 0;
+};
+
+struct GraphCache;  // batched.hip
+
+struct State {
+    std::mutex mu;
+    bool initialised = false;
+    int device = 0;
+    size_t pool_limit = 0;
+    bool pool_limit_explicit = false;
+    Slot slots[kNumSlots];
+    fh_stats stats{};
+    Pending pending;
+    GraphCache *graphs = nullptr;
+    // Phase timing (FERRIC_TIMING): events on the call's stream.
+    int timing = -1;  // -1: not yet read from the environment
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    fh_timing last_timing{};
+    size_t held_bytes() const;
+};
+
+// Phase markers: record(k) for k = 0 (start), 1 (after pack), 2 (after the
+// contraction), 3 (after the reductions); finish() after the stream sync.
+bool timing_on(State &s);
+void mark(State &s, int k, hipStream_t st);
+void finish_timing(State &s, uint64_t items);
+
+struct EnergyPlan {
+    int tile = 128;
+    int nvir_pad = 16;
+    int naux_pad = 128;
+    int ntile = 1;
+    int ntp = 1;
+    int64_t nitems = 0;
+};
+
+State &state();
+fh_status fail(fh_status st, const char *fmt, ...);
+void clear_error();
+fh_status ensure_device(State &s);
+fh_status reserve(State &s, const std::vector<std::pair<int, size_t>> &req);
+void release_all(State &s);
+fh_status hip_check(hipError_t e, const char *what);
+fh_status check_shapes(size_t naux, size_t nocc, size_t nvir, size_t n_eps, size_t first_occ,
+                       size_t nocc_total, bool *empty);
+fh_status check_kappa(double kappa, bool *has);
+fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t naux,
+                      hipStream_t stream);
+EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec);
+int tuned_tile(Prec prec, size_t nvir, size_t naux);
+size_t packed_elem_bytes(Prec p);
+fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux, size_t nocc,
+                      size_t nvir, const double *eps, size_t first_occ, size_t nocc_total,
+                      double kappa, Prec prec, hipStream_t stream, fh_spin_components *out,
+                      double *row_partials, const EnergyPlan *forced);
+void commit(State &s, Prec prec, bool adjoint, bool batched);
+bool to_prec(int p, Prec *out);
+void release_graphs(State &s);
+
+}  // namespace fh

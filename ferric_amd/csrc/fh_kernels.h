@@ -1,0 +1,114 @@
+// Launch interfaces of the RI-MP2 kernels (host side) and the scalar format
+// helpers shared by host and device code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace fh {
+
+// ---- OCP FP8 E4M3 (bias 7, max 448, no infinities, NaN = S.1111.111) ------
+
+// Round-to-nearest-even conversion with saturation to +-448.
+__host__ __device__ inline uint8_t f32_to_e4m3(float x) {
+    uint32_t bits;
+    __builtin_memcpy(&bits, &x, 4);
+    const uint8_t sign = (uint8_t)((bits >> 24) & 0x80u);
+    const float a = fabsf(x);
+    if (!(a == a)) return 0x7F;           // NaN
+    if (a >= 448.0f) return sign | 0x7E;  // saturate
+    if (a < 0.015625f) {                  // below 2^-6: subnormal step 2^-9
+        const float k = rintf(a * 512.0f);  // 0..8; 8 is the smallest normal, code 0x08
+        return sign | (uint8_t)k;
+    }
+    int e2;
+    (void)frexpf(a, &e2);
+    int ex = e2 - 1;                          // a in [2^ex, 2^(ex+1))
+    float q = rintf(ldexpf(a, 3 - ex));      // 8..16
+    if (q >= 16.0f) {
+        q = 8.0f;
+        ex += 1;
+    }
+    uint8_t code = (uint8_t)(((ex + 7) << 3) | ((int)q - 8));
+    if (code >= 0x7F) code = 0x7E;
+    return sign | code;
+}
+
+__host__ __device__ inline float e4m3_to_f32(uint8_t c) {
+    const float s = (c & 0x80) ? -1.0f : 1.0f;
+    const int e = (c >> 3) & 0xF, m = c & 7;
+    if (e == 15 && m == 7) return NAN;
+    if (e == 0) return s * ldexpf((float)m, -9);
+    return s * ldexpf((float)(8 + m), e - 10);
+}
+
+// Smallest e with 448 * 2^e >= amax (0 for amax == 0), clamped to E8M0's range.
+__host__ __device__ inline int fp8_scale_exponent(float amax) {
+    if (!(amax > 0.0f)) return 0;
+    int e2;
+    (void)frexpf(amax / 448.0f, &e2);
+    int e = e2;
+    while (e > -127 && ldexpf(448.0f, e - 1) >= amax) --e;
+    while (e < 127 && ldexpf(448.0f, e) < amax) ++e;
+    return e;
+}
+
+// ---- Packing ---------------------------------------------------------------
+
+enum class PackFmt { F64, F32, BF16X2, FP8 };
+
+struct PackArgs {
+    const void *b;    // caller layout, row-major (naux, nocc*nvir)
+    bool in_f32;      // element type of b
+    int naux, nocc, nvir, nvir_pad, naux_pad;
+    PackFmt fmt;
+    void *out0;       // F64/F32/FP8 rows, or BF16 hi
+    void *out1;       // BF16 lo
+    uint8_t *scale;   // FP8: (nocc, naux_pad/64) E8M0 bytes
+    double *lo_sq;    // BF16: per packed row, sum over Q of lo^2 (may be null)
+};
+
+hipError_t launch_pack(const PackArgs &a, hipStream_t st);
+hipError_t launch_absmax(const void *b, bool in_f32, int64_t n, unsigned int *out_bits,
+                         hipStream_t st);
+
+// ---- Energy ----------------------------------------------------------------
+
+enum class Prec { F64 = 0, F32 = 1, BF16X3 = 2, FP8 = 3 };
+
+// Panel depth along Q of each precision's kernel (naux_pad is a multiple).
+inline int panel_depth(Prec p) {
+    switch (p) {
+    case Prec::F64: return 8;
+    case Prec::F32: return 16;
+    case Prec::BF16X3: return 32;
+    case Prec::FP8: return 128;
+    }
+    return 128;
+}
+
+struct EnergyArgs {
+    Prec prec;
+    int tile;                 // 32, 64 or 128
+    const void *op0;          // packed rows (F64/F32/FP8) or BF16 hi
+    const void *op1;          // BF16 lo
+    const uint8_t *scale;     // FP8 scales (nocc, nqb)
+    const double *diag_corr;  // BF16: sum_Q lo^2 per packed row, added to K_ii[a, a]
+    int nocc, nvir, nvir_pad, naux_pad, nqb;
+    const double *eps_occ;    // device, active occupied (nocc)
+    const double *eps_vir;    // device, virtual (nvir)
+    double kappa;             // NaN = none
+    int ntile, ntp;           // tiles over nvir, tile pairs (tA <= tB)
+    int64_t nitems;           // npairs * ntp
+    double *item_partials;    // 4 doubles per item
+    double *row_partials;     // 4 doubles per i
+    double *result;           // e_os, e_ss, then 2*nocc row values
+};
+
+hipError_t launch_energy(const EnergyArgs &a, hipStream_t st);
+hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
+// Row reductions and the ascending fold into a.result.
+hipError_t launch_energy_reduce(const EnergyArgs &a, hipStream_t st);
+
+}  // namespace fh

@@ -1,0 +1,307 @@
+// Operand packing for the RI-MP2 contraction kernels (gfx950).
+//
+// The caller's tensor is b_ov[Q][i*nvir + a] (row-major (naux, nocc*nvir),
+// the reference layout, crates/ferric-mp2/src/rimp2.rs:1445-1499). The
+// contraction runs over Q, so every kernel reads "K-major" rows: one row per
+// (i, a) holding all Q contiguously. pack_* writes
+//
+//   bp[(i*nvir_pad + a)*naux_pad + Q]
+//
+// with nvir_pad = round_up(nvir, 16) and naux_pad = round_up(naux, KP) of the
+// consuming kernel; padding rows and columns are written as zeros, so no
+// memset is needed and the kernels' K loops need no bounds checks.
+//
+// Formats: FP64, FP32, BF16 hi/lo split (hi = RN(x), lo = RN(x - hi)), and
+// OCP FP8 E4M3 with one power-of-two (E8M0) scale per (occupied i, 64-row Q
+// block).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "fh_device.h"
+#include "fh_kernels.h"
+
+namespace fh {
+
+namespace {
+
+constexpr int PT = 64;  // transpose tile: 64 Q x 64 output rows
+
+template <typename Tin>
+__device__ __forceinline__ float load_as_float(const Tin *p) {
+    return (float)(*p);
+}
+
+// One 64x64 tile: rows r in [r0, r0+64) of the packed layout, Q in [q0, q0+64).
+// Reads are coalesced along ia (input rows), writes along Q (output rows).
+template <typename Tin, typename Tout, typename Conv>
+__device__ __forceinline__ void transpose_tile(const Tin *__restrict__ b, int64_t nov, int naux,
+                                               int nvir, int nvir_pad, int64_t nrows,
+                                               int naux_pad, Conv conv, Tout *__restrict__ out0,
+                                               Tout *__restrict__ out1) {
+    __shared__ float tile[PT][PT + 1];
+    const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;  // 64 x 4
+    const int64_t r0 = (int64_t)blockIdx.x * PT;
+    const int q0 = blockIdx.y * PT;
+    // Load: tile[q][r] = b[q0+q][col(r0+r)].
+    const int64_t r = r0 + tx;
+    int64_t col = -1;
+    if (r < nrows) {
+        const int64_t i = r / nvir_pad;
+        const int a = (int)(r % nvir_pad);
+        if (a < nvir) col = i * nvir + a;
+    }
+    for (int q = ty; q < PT; q += 4) {
+        const int qq = q0 + q;
+        float v = 0.0f;
+        if (col >= 0 && qq < naux) v = load_as_float(b + (int64_t)qq * nov + col);
+        tile[q][tx] = v;
+    }
+    __syncthreads();
+    // Store: out[(r0+rr)*naux_pad + q0 + tx] = tile[tx][rr].
+    for (int rr = ty; rr < PT; rr += 4) {
+        const int64_t orow = r0 + rr;
+        if (orow < nrows && q0 + tx < naux_pad) {
+            const int64_t o = orow * naux_pad + q0 + tx;
+            conv(tile[tx][rr], out0 + o, out1 ? out1 + o : nullptr);
+        }
+    }
+}
+
+// FP64 is transposed in double (no float round trip).
+template <typename Tin>
+__global__ __launch_bounds__(256) void pack_f64_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                       int naux, int nvir, int nvir_pad,
+                                                       int64_t nrows, int naux_pad,
+                                                       double *__restrict__ out) {
+    __shared__ double tile[PT][PT + 1];
+    const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;
+    const int64_t r0 = (int64_t)blockIdx.x * PT;
+    const int q0 = blockIdx.y * PT;
+    const int64_t r = r0 + tx;
+    int64_t col = -1;
+    if (r < nrows) {
+        const int64_t i = r / nvir_pad;
+        const int a = (int)(r % nvir_pad);
+        if (a < nvir) col = i * nvir + a;
+    }
+    for (int q = ty; q < PT; q += 4) {
+        const int qq = q0 + q;
+        double v = 0.0;
+        if (col >= 0 && qq < naux) v = (double)b[(int64_t)qq * nov + col];
+        tile[q][tx] = v;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < PT; rr += 4) {
+        const int64_t orow = r0 + rr;
+        if (orow < nrows && q0 + tx < naux_pad) out[orow * naux_pad + q0 + tx] = tile[tx][rr];
+    }
+}
+
+struct ConvF32 {
+    __device__ void operator()(float x, float *o, float *) const { *o = x; }
+};
+
+struct ConvBf16Split {
+    __device__ void operator()(float x, uint16_t *hi, uint16_t *lo) const {
+        __bf16 h = (__bf16)x;  // round to nearest even
+        float hf = (float)h;
+        __bf16 l = (__bf16)(x - hf);  // x - hf is exact (Sterbenz)
+        *hi = __builtin_bit_cast(uint16_t, h);
+        *lo = __builtin_bit_cast(uint16_t, l);
+    }
+};
+
+template <typename Tin>
+__global__ __launch_bounds__(256) void pack_f32_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                       int naux, int nvir, int nvir_pad,
+                                                       int64_t nrows, int naux_pad,
+                                                       float *__restrict__ out) {
+    transpose_tile<Tin, float>(b, nov, naux, nvir, nvir_pad, nrows, naux_pad, ConvF32{}, out,
+                               (float *)nullptr);
+}
+
+template <typename Tin>
+__global__ __launch_bounds__(256) void pack_bf16x3_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                          int naux, int nvir, int nvir_pad,
+                                                          int64_t nrows, int naux_pad,
+                                                          uint16_t *__restrict__ hi,
+                                                          uint16_t *__restrict__ lo) {
+    transpose_tile<Tin, uint16_t>(b, nov, naux, nvir, nvir_pad, nrows, naux_pad, ConvBf16Split{},
+                                  hi, lo);
+}
+
+// Per (i, 64-row Q block) maximum |B| over finite entries -> E8M0 scale
+// exponent byte e + 127 with 2^e >= amax / 448 (so every scaled value fits
+// E4M3's finite range). Grid (nqb, nocc), 256 threads.
+template <typename Tin>
+__global__ __launch_bounds__(256) void fp8_scale_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                        int naux, int nvir, int nqb,
+                                                        uint8_t *__restrict__ scale) {
+    const int qb = blockIdx.x, i = blockIdx.y;
+    float m = 0.0f;
+    for (int q = qb * 64; q < qb * 64 + 64 && q < naux; ++q) {
+        const Tin *row = b + (int64_t)q * nov + (int64_t)i * nvir;
+        for (int a = threadIdx.x; a < nvir; a += 256) {
+            float v = fabsf((float)row[a]);
+            if (v <= 3.0e38f && v > m) m = v;  // finite only
+        }
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scale[(int64_t)i * nqb + qb] = (uint8_t)(fp8_scale_exponent(red[0]) + 127);
+    }
+}
+
+template <typename Tin>
+__global__ __launch_bounds__(256) void pack_fp8_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                       int naux, int nvir, int nvir_pad,
+                                                       int64_t nrows, int naux_pad, int nqb,
+                                                       const uint8_t *__restrict__ scale,
+                                                       uint8_t *__restrict__ out) {
+    __shared__ float tile[PT][PT + 1];
+    const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;
+    const int64_t r0 = (int64_t)blockIdx.x * PT;
+    const int q0 = blockIdx.y * PT;  // one 64-row Q block per tile
+    const int64_t r = r0 + tx;
+    int64_t col = -1;
+    if (r < nrows) {
+        const int64_t i = r / nvir_pad;
+        const int a = (int)(r % nvir_pad);
+        if (a < nvir) col = i * nvir + a;
+    }
+    for (int q = ty; q < PT; q += 4) {
+        const int qq = q0 + q;
+        float v = 0.0f;
+        if (col >= 0 && qq < naux) v = (float)b[(int64_t)qq * nov + col];
+        tile[q][tx] = v;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < PT; rr += 4) {
+        const int64_t orow = r0 + rr;
+        if (orow < nrows && q0 + tx < naux_pad) {
+            const int64_t i = orow / nvir_pad;
+            const int e = (int)scale[i * nqb + blockIdx.y] - 127;
+            out[orow * naux_pad + q0 + tx] = f32_to_e4m3(ldexpf(tile[tx][rr], -e));
+        }
+    }
+}
+
+// Row sums of lo^2 over Q (FP64; each lo^2 is exact in FP32): the lo*lo term
+// the three-pass BF16 product omits. On the diagonal elements
+// K_ii[a, a] = sum_Q B^Q_ia^2 every omitted term is positive, so the omission
+// is a systematic bias there (elsewhere it has either sign and averages out);
+// the energy epilogue adds these sums back on exactly those elements.
+// One wave per packed row, 4 rows per workgroup.
+__global__ __launch_bounds__(256) void lo_sq_kernel(const uint16_t *__restrict__ lo,
+                                                    int64_t nrows, int naux_pad,
+                                                    double *__restrict__ out) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= nrows) return;
+    const uint16_t *p = lo + row * naux_pad;
+    double s = 0.0;
+    for (int q = lane; q < naux_pad; q += kWave) {
+        const float v = (float)__builtin_bit_cast(__bf16, p[q]);
+        s += (double)(v * v);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (lane == 0) out[row] = s;
+}
+
+// max over finite |x| (for the reduced-precision range check).
+template <typename Tin>
+__global__ __launch_bounds__(256) void absmax_kernel(const Tin *__restrict__ b, int64_t n,
+                                                     unsigned int *__restrict__ out_bits) {
+    float m = 0.0f;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256) {
+        double v = fabs((double)b[k]);
+        if (v <= 1.7e308 && (float)v > m) m = (float)fmin(v, 3.0e38);
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    // Non-negative floats order like their bit patterns: an integer max is exact.
+    if (threadIdx.x == 0) atomicMax(out_bits, __float_as_uint(red[0]));
+}
+
+__global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
+
+}  // namespace
+
+static inline dim3 pack_grid(int64_t nrows, int naux_pad) {
+    return dim3((unsigned)((nrows + PT - 1) / PT), (unsigned)((naux_pad + PT - 1) / PT));
+}
+
+hipError_t launch_pack(const PackArgs &a, hipStream_t st) {
+    const int64_t nrows = (int64_t)a.nocc * a.nvir_pad;
+    const int64_t nov = (int64_t)a.nocc * a.nvir;
+    dim3 grid = pack_grid(nrows, a.naux_pad);
+    const bool f32in = a.in_f32;
+#define FH_LAUNCH(K, ...)                                                                      \
+    do {                                                                                       \
+        if (f32in)                                                                             \
+            hipLaunchKernelGGL(K<float>, grid, dim3(256), 0, st, (const float *)a.b, nov,      \
+                               a.naux, a.nvir, a.nvir_pad, nrows, a.naux_pad, __VA_ARGS__);    \
+        else                                                                                   \
+            hipLaunchKernelGGL(K<double>, grid, dim3(256), 0, st, (const double *)a.b, nov,    \
+                               a.naux, a.nvir, a.nvir_pad, nrows, a.naux_pad, __VA_ARGS__);    \
+    } while (0)
+    switch (a.fmt) {
+    case PackFmt::F64:
+        FH_LAUNCH(pack_f64_kernel, (double *)a.out0);
+        break;
+    case PackFmt::F32:
+        FH_LAUNCH(pack_f32_kernel, (float *)a.out0);
+        break;
+    case PackFmt::BF16X2:
+        FH_LAUNCH(pack_bf16x3_kernel, (uint16_t *)a.out0, (uint16_t *)a.out1);
+        if (a.lo_sq)
+            hipLaunchKernelGGL(lo_sq_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, st,
+                               (const uint16_t *)a.out1, nrows, a.naux_pad, a.lo_sq);
+        break;
+    case PackFmt::FP8: {
+        const int nqb = a.naux_pad / 64;
+        dim3 sg((unsigned)nqb, (unsigned)a.nocc);
+        if (f32in)
+            hipLaunchKernelGGL(fp8_scale_kernel<float>, sg, dim3(256), 0, st, (const float *)a.b,
+                               nov, a.naux, a.nvir, nqb, a.scale);
+        else
+            hipLaunchKernelGGL(fp8_scale_kernel<double>, sg, dim3(256), 0, st,
+                               (const double *)a.b, nov, a.naux, a.nvir, nqb, a.scale);
+        FH_LAUNCH(pack_fp8_kernel, nqb, (const uint8_t *)a.scale, (uint8_t *)a.out0);
+        break;
+    }
+    }
+#undef FH_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(const void *b, bool in_f32, int64_t n, unsigned int *out_bits,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, st, out_bits);
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    if (in_f32)
+        hipLaunchKernelGGL(absmax_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, st,
+                           (const float *)b, n, out_bits);
+    else
+        hipLaunchKernelGGL(absmax_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, st,
+                           (const double *)b, n, out_bits);
+    return hipGetLastError();
+}
+
+}  // namespace fh

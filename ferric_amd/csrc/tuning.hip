@@ -1,0 +1,49 @@
+// Tile selection for the pair-energy kernels.
+//
+// A tile table (written by ferric_amd/autotune.py and installed through
+// fh_tune_set) maps (precision, nvir bucket, naux bucket) to the measured best
+// tile side; FERRIC_HIP_TILE forces one tile for every call.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <mutex>
+#include <vector>
+
+#include "fh_host.h"
+
+namespace fh {
+
+namespace {
+struct TuneEntry {
+    int prec;
+    size_t nvir_max, naux_max;
+    int tile;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_table;
+}  // namespace
+
+int tuned_tile(Prec prec, size_t nvir, size_t naux) {
+    const char *env = getenv("FERRIC_HIP_TILE");
+    if (env && *env) {
+        int t = atoi(env);
+        if (t == 32 || t == 64 || t == 128) return t;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    // First entry (in installation order) whose bucket holds the shape.
+    for (const auto &e : g_table)
+        if (e.prec == (int)prec && nvir <= e.nvir_max && naux <= e.naux_max) return e.tile;
+    return 0;
+}
+
+void tune_set(int prec, size_t nvir_max, size_t naux_max, int tile) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_table.push_back(TuneEntry{prec, nvir_max, naux_max, tile});
+}
+
+void tune_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_table.clear();
+}
+
+}  // namespace fh

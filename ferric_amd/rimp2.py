@@ -1,0 +1,198 @@
+"""Host mirror of ferric's RI-MP2 pair-energy surface over the HIP library.
+
+Names, argument meaning and error behaviour follow crates/ferric-mp2/src/rimp2.rs:
+
+* ``spin_components_from_b_ov``          rimp2.rs:1311-1320 (FP64 device)
+* ``spin_components_from_b_ov_kappa``    rimp2.rs:1330-1341 (may use reduced precision)
+* ``spin_components_from_b_ov_kappa_f64`` rimp2.rs:1343-1357 (FP64 device)
+* ``try_spin_components_on_device``      rimp2_gpu.rs:105-170 (None means "run the CPU path")
+* ``active_occ``                          rimp2.rs:99-106
+* ``scs_mp2`` / ``ScsMp2Config``          crates/ferric-mp2/src/scs.rs:27-49, :93-114
+
+``b_ov`` is row-major (naux, nocc*nvir) over ACTIVE occupied orbitals; ``eps`` is
+the full MO energy vector (occupied i at eps[first_occ + i], virtual a at
+eps[nocc_total + a]). ``b_ov`` may be a host numpy array (FP64; uploaded for the
+call, as the reference's Stage does) or a device torch tensor (FP64 or FP32;
+used in place). Every call runs on the GPU; failures raise the GpuError
+hierarchy of :mod:`ferric_amd._lib` -- this package has no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import GpuError, LayoutError, Precision, check, lib
+
+__all__ = [
+    "SpinComponents", "Precision", "active_occ", "spin_components_from_b_ov",
+    "spin_components_from_b_ov_kappa", "spin_components_from_b_ov_kappa_f64",
+    "try_spin_components_on_device", "spin_components", "ScsMp2Config", "scs_mp2",
+    "settings_precision",
+]
+
+
+@dataclass(frozen=True)
+class SpinComponents:
+    """Spin-component resolved MP2 correlation energy (rimp2.rs:134-141)."""
+
+    e_os: float
+    e_ss: float
+    e_total: float
+
+
+def active_occ(nocc_total: int, frozen_core: int) -> int:
+    """Active occupied count; an error when the freeze covers every occupied
+    orbital (``frozen_core == 0`` with zero occupied is allowed), rimp2.rs:99-106."""
+    if frozen_core != 0 and frozen_core >= nocc_total:
+        raise LayoutError(
+            f"frozen_core = {frozen_core} freezes all {nocc_total} occupied orbitals — "
+            "nothing left to correlate")
+    return nocc_total - frozen_core
+
+
+_PREC_NAMES = {
+    "f64": Precision.FP64, "fp64": Precision.FP64,
+    "f32": Precision.FP32, "fp32": Precision.FP32, "mixed": Precision.FP32,
+    "bf16x3": Precision.BF16X3, "bf16": Precision.BF16X3,
+    "fp8": Precision.FP8_E4M3, "fp8_e4m3": Precision.FP8_E4M3,
+}
+
+
+def settings_precision() -> Precision:
+    """Reduced precision allowed for the energy kernel by the process settings.
+
+    ``FERRIC_GPU_PRECISION`` (f64 | mixed | fp32 | bf16x3 | fp8), the reference's
+    precision knob (crates/ferric-core/src/gpu/precision.rs:283); default f64.
+    A malformed value is an error, as with the reference's ConfigVar.
+    """
+    v = os.environ.get("FERRIC_GPU_PRECISION", "f64").strip().lower()
+    if v not in _PREC_NAMES:
+        raise LayoutError(f"FERRIC_GPU_PRECISION={v!r} is not one of {sorted(_PREC_NAMES)}")
+    return _PREC_NAMES[v]
+
+
+def _kappa_c(kappa) -> float:
+    return float("nan") if kappa is None else float(kappa)
+
+
+def _is_torch_cuda(x) -> bool:
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+def spin_components(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
+                    nocc_total: int | None = None, kappa=None,
+                    precision: Precision = Precision.FP64, row_partials: bool = False):
+    """The device energy. Returns SpinComponents (and the per-i partials when
+    ``row_partials``). Raises GpuError subclasses on failure."""
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    precision = Precision(precision)
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    out = _lib.SpinComponentsC()
+    L = lib()
+    if _is_torch_cuda(b_ov):
+        import torch
+
+        if b_ov.dtype == torch.float64:
+            dt = _lib.DType.F64
+        elif b_ov.dtype == torch.float32:
+            dt = _lib.DType.F32
+        else:
+            raise LayoutError(f"b_ov dtype {b_ov.dtype} is not float64/float32")
+        if b_ov.dim() != 2 or not b_ov.is_contiguous():
+            raise LayoutError("b_ov must be a contiguous 2-D (naux, nocc*nvir) tensor")
+        naux, width = b_ov.shape
+        if width != nocc * nvir:
+            raise LayoutError(f"b_ov is {naux}x{width} but nocc*nvir = {nocc * nvir}")
+        rows = np.zeros(2 * max(nocc, 1))
+        stream = torch.cuda.current_stream(b_ov.device).cuda_stream
+        st = L.fh_rimp2_energy_dev(
+            ctypes.c_void_p(b_ov.data_ptr()), int(dt), naux, nocc, nvir,
+            e.ctypes.data_as(_lib._c_dp), e.size, first_occ, nocc_total, _kappa_c(kappa),
+            int(precision), ctypes.c_void_p(stream), ctypes.byref(out),
+            rows.ctypes.data_as(_lib._c_dp))
+        check(st, "fh_rimp2_energy_dev")
+        sc = SpinComponents(out.e_os, out.e_ss, out.e_total)
+        if row_partials:
+            return sc, rows[:2 * nocc].reshape(nocc, 2)
+        return sc
+    b = np.ascontiguousarray(np.asarray(b_ov, dtype=np.float64))
+    if b.ndim != 2:
+        raise LayoutError("b_ov must be 2-D (naux, nocc*nvir)")
+    naux, width = b.shape
+    if width != nocc * nvir:
+        raise LayoutError(f"b_ov is {naux}x{width} but nocc*nvir = {nocc * nvir}")
+    if row_partials:
+        import torch
+
+        bt = torch.from_numpy(b).cuda()
+        return spin_components(bt, e, nocc, nvir, first_occ, nocc_total, kappa, precision, True)
+    st = L.fh_rimp2_energy(b.ctypes.data_as(_lib._c_dp), naux, nocc, nvir,
+                           e.ctypes.data_as(_lib._c_dp), e.size, first_occ, nocc_total,
+                           _kappa_c(kappa), int(precision), ctypes.byref(out))
+    check(st, "fh_rimp2_energy")
+    return SpinComponents(out.e_os, out.e_ss, out.e_total)
+
+
+def _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, mixed_ok) -> SpinComponents:
+    """dispatch_spin_components (rimp2.rs:1360-1375) minus the CPU branch: the
+    reduced-precision kernel only when ``mixed_ok`` and the settings allow it."""
+    prec = settings_precision() if mixed_ok else Precision.FP64
+    try:
+        return spin_components(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, prec)
+    except _lib.RangeError:
+        if prec == Precision.FP64:
+            raise
+        # Reduced precision refused (operand range): FP64 on the device, as
+        # the reference's mixed -> f64 fallback (rimp2_gpu.rs:149-161).
+        return spin_components(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa,
+                               Precision.FP64)
+
+
+def spin_components_from_b_ov(b_ov, eps, nocc, nvir, first_occ, nocc_total) -> SpinComponents:
+    """rimp2.rs:1311-1320: FP64 device, plain MP2."""
+    return _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, None, False)
+
+
+def spin_components_from_b_ov_kappa(b_ov, eps, nocc, nvir, first_occ, nocc_total,
+                                    kappa=None) -> SpinComponents:
+    """rimp2.rs:1330-1341: reduced precision allowed by the settings."""
+    return _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, True)
+
+
+def spin_components_from_b_ov_kappa_f64(b_ov, eps, nocc, nvir, first_occ, nocc_total,
+                                        kappa=None) -> SpinComponents:
+    """rimp2.rs:1343-1357: the device stays FP64."""
+    return _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, False)
+
+
+def try_spin_components_on_device(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa=None,
+                                  mixed_ok: bool = False):
+    """rimp2_gpu.rs:105-170: SpinComponents, or None when the caller should run
+    its own CPU path (every GpuError, including a missing device)."""
+    if os.environ.get("FERRIC_GPU", "auto").strip().lower() == "off":
+        return None
+    try:
+        return _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, mixed_ok)
+    except GpuError:
+        return None
+
+
+@dataclass(frozen=True)
+class ScsMp2Config:
+    """SCS coefficients (crates/ferric-mp2/src/scs.rs:27-49; Grimme's 6/5, 1/3)."""
+
+    c_os: float = 6.0 / 5.0
+    c_ss: float = 1.0 / 3.0
+
+
+def scs_mp2(sc: SpinComponents, cfg: ScsMp2Config = ScsMp2Config()) -> float:
+    """E = c_os E_os + c_ss E_ss, applied after the kernel (scs.rs:105-106)."""
+    if not (math.isfinite(cfg.c_os) and math.isfinite(cfg.c_ss)):
+        raise LayoutError("SCS coefficients must be finite")
+    return cfg.c_os * sc.e_os + cfg.c_ss * sc.e_ss

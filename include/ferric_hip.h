@@ -1,0 +1,230 @@
+/*
+ * ferric_hip.h -- C ABI of the MI355X (gfx950) RI-MP2 pair-energy library.
+ *
+ * Drop-in for the device boundary of ferric's closed-shell RI-MP2 stage:
+ *
+ *   try_spin_components_on_device(b_ov, eps, nocc, nvir, first_occ,
+ *                                 nocc_total, kappa, mixed_ok)
+ *       -> Option<SpinComponents>
+ *   (crates/ferric-mp2/src/rimp2_gpu.rs:105-114; called by
+ *    dispatch_spin_components, crates/ferric-mp2/src/rimp2.rs:1360-1375)
+ *
+ * Conventions kept from the reference:
+ *   - b_ov is row-major (naux, nocc*nvir): b_ov[P*nocc*nvir + i*nvir + a] = B^P_ia,
+ *     over ACTIVE occupied i (rimp2.rs:1445-1499).
+ *   - eps is the full MO energy vector: occupied i at eps[first_occ + i],
+ *     virtual a at eps[nocc_total + a]; first_occ is the frozen-core count
+ *     (rimp2.rs:1399-1441, :99-106).
+ *   - Shape checks first, in the reference's order (rimp2_gpu.rs:211-237):
+ *     nocc*nvir overflow, eps too short -> FH_LAYOUT; nocc == 0 || nvir == 0
+ *     -> zeros and FH_OK; naux == 0 -> FH_LAYOUT (the CPU path owns it).
+ *   - kappa: NaN means None (plain MP2); otherwise it must be finite and > 0
+ *     (rimp2.rs:1266-1272) and every term is damped by (1 - e^{kappa D})^2.
+ *   - Never aborts. Every failure is a typed status mirroring GpuError
+ *     (crates/ferric-core/src/gpu/device.rs:16-36); the caller runs its own
+ *     CPU path on any status other than FH_OK. This library has no CPU path.
+ *   - Calls are serialised per device (one mutex per device). The library
+ *     allocates its device workspace itself against a pool ceiling
+ *     (FERRIC_GPU_MEM_GB, default 80% of free device memory,
+ *     crates/ferric-core/src/gpu/mod.rs:348-353) and refuses with
+ *     FH_POOL_FULL before any transfer when a call would exceed it.
+ *
+ * Every pointer argument is a plain host or device pointer as stated per
+ * entry point; no entry point takes or returns a framework type.
+ */
+#ifndef FERRIC_HIP_H
+#define FERRIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FERRIC_HIP_ABI_VERSION 1
+
+/* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
+typedef enum fh_status {
+    FH_OK = 0,
+    FH_LAYOUT = 1,      /* GpuError::Layout: shapes/offsets inconsistent */
+    FH_POOL_FULL = 2,   /* GpuError::PoolFull: workspace exceeds the pool ceiling */
+    FH_DEVICE = 3,      /* GpuError::Device: no usable gfx950 device */
+    FH_HIP = 4,         /* GpuError::Cuda: a HIP runtime call failed */
+    FH_KERNEL = 5,      /* GpuError::Kernel: a kernel could not be launched */
+    FH_RANGE = 6,       /* GpuError::F32Range: operand outside the reduced format's range */
+    FH_UNSUPPORTED = 7, /* option outside this library's surface (e.g. kappa in the adjoint) */
+} fh_status;
+
+/* Arithmetic of the (ia|jb) contraction. The orbital-energy denominators and
+ * the pair-energy sums are FP64 (compensated) in every mode.
+ *   FH_FP64     FP64 MFMA, FP64 accumulation (the plain method)
+ *   FH_FP32     FP32 operands, FP32 MFMA, FP32 accumulation
+ *   FH_BF16X3   BF16 hi/lo split, hi*hi + hi*lo + lo*hi, FP32 accumulation
+ *   FH_FP8_E4M3 OCP E4M3 operands with one power-of-two scale per
+ *               (64-row Q block, occupied i), FP32 accumulation
+ * The reference's Precision::{F64, Mixed} (crates/ferric-core/src/gpu/precision.rs)
+ * map to FH_FP64 and FH_FP32. */
+typedef enum fh_precision {
+    FH_FP64 = 0,
+    FH_FP32 = 1,
+    FH_BF16X3 = 2,
+    FH_FP8_E4M3 = 3,
+} fh_precision;
+
+/* Element type of a device-resident b_ov. */
+typedef enum fh_dtype {
+    FH_DT_F64 = 0,
+    FH_DT_F32 = 1,
+} fh_dtype;
+
+/* SpinComponents (crates/ferric-mp2/src/rimp2.rs:134-141). */
+typedef struct fh_spin_components {
+    double e_os;
+    double e_ss;
+    double e_total; /* e_os + e_ss */
+} fh_spin_components;
+
+/* Counters in the spirit of GpuStatsSnapshot
+ * (crates/ferric-core/src/gpu/stats.rs:308-350). Committed only when a call
+ * succeeds as a whole, as the reference's Tally (rimp2_gpu.rs:86-100). */
+typedef struct fh_stats {
+    uint64_t calls;            /* entry-point calls */
+    uint64_t offloaded;        /* calls that returned FH_OK */
+    uint64_t fp64_calls;
+    uint64_t fp32_calls;
+    uint64_t bf16x3_calls;
+    uint64_t fp8_calls;
+    uint64_t adjoint_calls;
+    uint64_t batched_calls;
+    uint64_t h2d_bytes;
+    uint64_t d2h_bytes;
+    uint64_t kernel_launches;
+    uint64_t fail_layout;      /* failures per class (counted whether or not the call succeeded) */
+    uint64_t fail_pool_full;
+    uint64_t fail_hip;
+    uint64_t fail_range;
+    uint64_t fail_unsupported;
+    uint64_t pool_peak_bytes;  /* high-water mark of the library's workspace */
+} fh_stats;
+
+/* One local-MP2 pair domain for fh_rimp2_batched_dev: its B (row-major
+ * (naux, nocc*nvir)) starts at element b_offset of the packed B buffer and its
+ * orbital energies at eps_offset of the packed eps buffer, laid out
+ * [nocc occupied, then nvir virtual] (no frozen core inside a domain). */
+typedef struct fh_domain {
+    uint64_t b_offset;
+    uint64_t eps_offset;
+    uint32_t nocc;
+    uint32_t nvir;
+    uint32_t naux;
+    uint32_t reserved;
+} fh_domain;
+
+/* Phase times of the last successful call on this device, measured with HIP
+ * events on the call's stream (the reference's FERRIC_TIMING phases,
+ * crates/ferric-core/src/timing.rs:161). Recorded only while timing is on
+ * (fh_set_timing(1) or FERRIC_TIMING=1 at first use); otherwise zeros. */
+typedef struct fh_timing {
+    double pack_ms;      /* operand packing / quantisation */
+    double contract_ms;  /* fused contraction + pair-energy kernel(s) */
+    double reduce_ms;    /* deterministic reductions */
+    double total_ms;     /* first to last event */
+    uint64_t items;      /* work items of the contraction launch */
+} fh_timing;
+
+/* ---- Library state ---------------------------------------------------- */
+
+/* ABI version (FERRIC_HIP_ABI_VERSION). */
+int fh_abi_version(void);
+
+/* Select and initialise a device (install-once settings,
+ * crates/ferric-core/src/gpu/mod.rs:145-207). device < 0 takes
+ * FERRIC_GPU_DEVICE or 0. Idempotent per device. */
+fh_status fh_init(int device);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *fh_last_error(void);
+
+/* Pool ceiling in bytes for the library's device workspace (0 restores the
+ * default: FERRIC_GPU_MEM_GB if set, else 80% of free device memory). */
+fh_status fh_set_pool_limit(size_t bytes);
+fh_status fh_get_pool_limit(size_t *bytes);
+
+/* Free the cached device workspace (it is otherwise kept between calls). */
+fh_status fh_release_workspace(void);
+
+fh_status fh_get_stats(fh_stats *out);
+fh_status fh_reset_stats(void);
+
+/* Phase timing on/off (see fh_timing) and the last call's phase times. */
+fh_status fh_set_timing(int on);
+fh_status fh_last_timing(fh_timing *out);
+
+/* ---- Energy ------------------------------------------------------------ */
+
+/* The drop-in for try_spin_components_on_device (rimp2_gpu.rs:105-170):
+ * b_ov and eps are HOST pointers (borrowed, read-only); the library makes a
+ * resident device copy for the call (Stage, rimp2_gpu.rs:312-373). */
+fh_status fh_rimp2_energy(const double *b_ov, size_t naux, size_t nocc, size_t nvir,
+                          const double *eps, size_t n_eps, size_t first_occ,
+                          size_t nocc_total, double kappa, fh_precision precision,
+                          fh_spin_components *out);
+
+/* Same energy with b_ov already resident on the device (b_dev, element type
+ * b_dtype, row-major (naux, nocc*nvir)); eps stays a host pointer. Work is
+ * queued on `stream` (a hipStream_t, NULL = the null stream) and the call
+ * returns after the result is on the host. row_partials (host, 2*nocc doubles,
+ * may be NULL) receives the per-i partials (e_os_i, e_ss_i), each already
+ * carrying the pair weights, whose ascending sum is the energy
+ * (spin_components_from_blocks, rimp2.rs:1506-1544). */
+fh_status fh_rimp2_energy_dev(const void *b_dev, fh_dtype b_dtype, size_t naux, size_t nocc,
+                              size_t nvir, const double *eps, size_t n_eps, size_t first_occ,
+                              size_t nocc_total, double kappa, fh_precision precision,
+                              void *stream, fh_spin_components *out, double *row_partials);
+
+/* ---- Adjoint ----------------------------------------------------------- */
+
+/* E = c_os*E_os + c_ss*E_ss (SCS surface, crates/ferric-mp2/src/scs.rs:93-114)
+ * and its adjoint:
+ *   gamma[P*nov + i*nvir + a] = dE/dB^P_ia = 4 x[P, ia], with
+ *     x[P, ia] = sum_jb T~_ij^ab B^P_jb, T~ = (c_os + c_ss) t - c_ss t^T
+ *     (zvector.rs:651-697, rimp2.rs:1832-1840, :1987-2011);
+ *   deps[first_occ + k] = dE/deps_k = 2 P_kk (active occupied),
+ *   deps[nocc_total + c] = 2 P_cc (virtual), 0 elsewhere (frozen core)
+ *     (oo_rimp2.rs:1192-1300, rimp2.rs:2026-2052).
+ * T~ is recomputed per (i,j) block and never stored for the whole problem.
+ * kappa must be NaN (None): FH_UNSUPPORTED otherwise.
+ * Host-pointer version: b_ov, eps, gamma (naux*nov) and deps (n_eps) on the host. */
+fh_status fh_rimp2_energy_adjoint(const double *b_ov, size_t naux, size_t nocc, size_t nvir,
+                                  const double *eps, size_t n_eps, size_t first_occ,
+                                  size_t nocc_total, double c_os, double c_ss,
+                                  fh_precision precision, fh_spin_components *out,
+                                  double *gamma, double *deps);
+
+/* Device-resident version: b_dev (b_dtype) and gamma_dev (FP64, naux*nov,
+ * overwritten) on the device; eps and deps on the host. */
+fh_status fh_rimp2_energy_adjoint_dev(const void *b_dev, fh_dtype b_dtype, size_t naux,
+                                      size_t nocc, size_t nvir, const double *eps, size_t n_eps,
+                                      size_t first_occ, size_t nocc_total, double c_os,
+                                      double c_ss, fh_precision precision, void *stream,
+                                      fh_spin_components *out, double *gamma_dev, double *deps);
+
+/* ---- Batched local-MP2 domains ---------------------------------------- */
+
+/* Energies of ndom independent pair domains in one persistent launch
+ * (the serial pair/domain loop it replaces: crates/ferric-mp2/src/dlpno_mp2.rs:718-745).
+ * domains is a HOST array; b_dev (FP64) and eps_dev (FP64) are the packed
+ * device buffers the descriptors point into; e_os/e_ss are HOST arrays of
+ * ndom. precision: FH_FP64 or FH_BF16X3. use_graph != 0 captures the launch
+ * sequence in a hipGraph on first use for this descriptor set and replays it. */
+fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom, const double *b_dev,
+                               size_t b_len, const double *eps_dev, size_t eps_len,
+                               fh_precision precision, int use_graph, void *stream,
+                               double *e_os, double *e_ss);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FERRIC_HIP_H */

@@ -1,0 +1,244 @@
+"""CPU oracle for the RI-MP2 pair-energy stage (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+import this package, and only as the checker or the timed CPU reference. The
+product (``ferric_amd``) never imports it: there is no CPU fallback.
+
+``rimp2_oracle.c`` restates the reference's algorithm loop for loop (file:line
+citations in its header); this module is a thin ctypes binding plus two drivers:
+
+* :func:`spin_components` -- the C oracle end to end (own blocked GEMM);
+* :func:`spin_components_blas` -- the reference's CPU algorithm with the tail
+  block ``G_i = B_i^T B_tail`` formed by OpenBLAS (numpy's BLAS, the library the
+  reference links), one task per ``i`` on a thread pool with single-threaded
+  BLAS inside tasks, the C pair loop, and the i-ordered serial fold
+  (crates/ferric-mp2/src/rimp2.rs:1445-1544). This is the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "librimp2_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_sz = ctypes.c_size_t
+
+
+def build(force: bool = False) -> str:
+    """Compile the C oracle with gcc (``oracle/Makefile``)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_active_occ.argtypes = [_sz, _sz, ctypes.POINTER(_sz)]
+        L.orc_tail_block.argtypes = [_dp, _sz, _sz, _sz, _sz, _dp]
+        L.orc_pair_energy.argtypes = [_dp, _sz, _sz, _sz, _sz, _dp, _sz, _sz, ctypes.c_int,
+                                      ctypes.c_double, _dp, _dp]
+        L.orc_i_partial.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, ctypes.c_int,
+                                    ctypes.c_double, _dp, _dp]
+        L.orc_spin_components_from_b_ov.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, _sz,
+                                                    ctypes.c_int, ctypes.c_double,
+                                                    ctypes.c_int, _dp]
+        L.orc_spin_components_from_g.argtypes = [_dp, _sz, _sz, _dp, _sz, _sz, _dp]
+        L.orc_t2_and_integrals.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, _dp, _dp]
+        L.orc_x_ov_weighted.argtypes = [_dp, _dp, _sz, _sz, _sz, ctypes.c_double,
+                                        ctypes.c_double, _dp]
+        L.orc_mp2_density.argtypes = [_dp, _sz, _sz, _dp, _dp]
+        L.orc_opposite_spin_density.argtypes = [_dp, _sz, _sz, _dp, _dp]
+        L.orc_scaled_density.argtypes = [_dp, _sz, _sz, ctypes.c_double, ctypes.c_double,
+                                         _dp, _dp]
+        L.orc_rimp2_adjoint.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, _sz,
+                                        ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_energy_bound.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def active_occ(nocc_total: int, frozen_core: int) -> int:
+    """rimp2.rs:99-106."""
+    out = _sz(0)
+    if lib().orc_active_occ(nocc_total, frozen_core, ctypes.byref(out)):
+        raise OracleError(f"frozen_core = {frozen_core} freezes all {nocc_total} occupied orbitals")
+    return out.value
+
+
+def _kappa_args(kappa):
+    return (0, 0.0) if kappa is None else (1, float(kappa))
+
+
+def spin_components(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, kappa=None,
+                    nthreads: int = 0):
+    """(e_os, e_ss, e_total) from the C oracle (rimp2.rs:1445-1544)."""
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    naux = b.shape[0] if b.ndim == 2 else 0
+    out = np.zeros(3)
+    hk, k = _kappa_args(kappa)
+    rc = lib().orc_spin_components_from_b_ov(_p(b), naux, nocc, nvir, _p(e), e.size, first_occ,
+                                             nocc_total, hk, k, nthreads, _p(out))
+    if rc:
+        raise OracleError(f"oracle error {rc}")
+    return tuple(float(x) for x in out)
+
+
+def tail_block(b_ov, nocc, nvir, i):
+    b = _f64(b_ov)
+    g = np.zeros((nvir, (nocc - i) * nvir))
+    lib().orc_tail_block(_p(b), b.shape[0], nocc, nvir, i, _p(g))
+    return g
+
+
+def i_partial(g_i, i, nocc, nvir, eps, first_occ, nocc_total, kappa=None):
+    g = _f64(g_i)
+    e = _f64(eps)
+    os_ = ctypes.c_double(0.0)
+    ss_ = ctypes.c_double(0.0)
+    hk, k = _kappa_args(kappa)
+    lib().orc_i_partial(_p(g), i, nocc, nvir, _p(e), first_occ, nocc_total, hk, k,
+                        ctypes.byref(os_), ctypes.byref(ss_))
+    return os_.value, ss_.value
+
+
+def spin_components_blas(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, kappa=None,
+                         nthreads: int | None = None, i_rows=None):
+    """The reference's CPU algorithm with OpenBLAS tail GEMMs (the timed baseline).
+
+    ``i_rows`` restricts the work to a subset of occupied rows (a bounded sample
+    for timing); the default is every row, which gives the full energy.
+    Returns (e_os, e_ss, e_total) over the rows done.
+    """
+    from threadpoolctl import threadpool_limits
+
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    if nthreads is None:
+        nthreads = len(os.sched_getaffinity(0))
+    rows = list(range(nocc)) if i_rows is None else list(i_rows)
+    parts = {}
+
+    def task(i):
+        bi = b[:, i * nvir:(i + 1) * nvir]
+        tail = b[:, i * nvir:]
+        g = np.ascontiguousarray(bi.T @ tail)
+        parts[i] = i_partial(g, i, nocc, nvir, e, first_occ, nocc_total, kappa)
+
+    with threadpool_limits(limits=1, user_api="blas"):
+        with ThreadPoolExecutor(max_workers=nthreads) as ex:
+            list(ex.map(task, rows))
+    os_ = ss_ = 0.0
+    for i in sorted(parts):
+        os_ += parts[i][0]
+        ss_ += parts[i][1]
+    return os_, ss_, os_ + ss_
+
+
+def spin_components_from_g(g, eps, nocc, nvir, first_occ=0, nocc_total=None):
+    """rimp2.rs:984-1018."""
+    gg = _f64(g)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    out = np.zeros(3)
+    lib().orc_spin_components_from_g(_p(gg), nocc, nvir, _p(e), first_occ, nocc_total, _p(out))
+    return tuple(float(x) for x in out)
+
+
+def t2_and_integrals(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None):
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    nov = nocc * nvir
+    t2 = np.zeros((nov, nov))
+    eri = np.zeros((nov, nov))
+    lib().orc_t2_and_integrals(_p(b), b.shape[0], nocc, nvir, _p(e), nocc_total, first_occ,
+                               _p(t2), _p(eri))
+    return t2, eri
+
+
+def x_ov_weighted(t2, b_ov, nocc, nvir, c_os=1.0, c_ss=1.0):
+    t = _f64(t2)
+    b = _f64(b_ov)
+    x = np.zeros_like(b)
+    rc = lib().orc_x_ov_weighted(_p(t), _p(b), nocc, nvir, b.shape[0], c_os, c_ss, _p(x))
+    if rc:
+        raise OracleError(f"oracle error {rc}")
+    return x
+
+
+def mp2_density(t2, nocc, nvir, c_os=1.0, c_ss=1.0):
+    t = _f64(t2)
+    p_oo = np.zeros((nocc, nocc))
+    p_vv = np.zeros((nvir, nvir))
+    lib().orc_scaled_density(_p(t), nocc, nvir, c_os, c_ss, _p(p_oo), _p(p_vv))
+    return p_oo, p_vv
+
+
+def adjoint(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, c_os=1.0, c_ss=1.0):
+    """(E_os, E_ss, E, Gamma = dE/dB, dE/deps) through the reference's intermediates."""
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    gamma = np.zeros_like(b)
+    deps = np.zeros_like(e)
+    out = np.zeros(3)
+    rc = lib().orc_rimp2_adjoint(_p(b), b.shape[0], nocc, nvir, _p(e), e.size, first_occ,
+                                 nocc_total, c_os, c_ss, _p(gamma), _p(deps), _p(out))
+    if rc:
+        raise OracleError(f"oracle error {rc}")
+    return out[0], out[1], out[2], gamma, deps
+
+
+def energy_bound(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, kappa=None,
+                 eps_g=None, eta=0.0):
+    """Worst-case |dE_os|, |dE_ss| for per-element G perturbations eps_g*S_ab + eta.
+
+    Default eps_g is the f64 device-vs-CPU factor 2*gamma_naux(u64).
+    """
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    naux = b.shape[0]
+    if eps_g is None:
+        u = 2.0 ** -53
+        eps_g = 2.0 * naux * u / (1.0 - naux * u)
+    out = np.zeros(2)
+    hk, k = _kappa_args(kappa)
+    lib().orc_energy_bound(_p(b), naux, nocc, nvir, _p(e), first_occ, nocc_total, hk, k,
+                           float(eps_g), float(eta), _p(out))
+    return float(out[0]), float(out[1])

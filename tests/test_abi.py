@@ -1,0 +1,109 @@
+"""The C ABI library loads and exports every symbol include/ferric_hip.h declares.
+
+These run without a GPU: only symbol lookup and the argument checks that the
+library performs before touching a device (the reference's shape checks come
+first, crates/ferric-mp2/src/rimp2_gpu.rs:211-237).
+"""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ferric_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ferric_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_table():
+    assert declared_functions() == sorted(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    assert L.fh_abi_version() == 1
+
+
+def test_enum_values_match_header():
+    text = open(HEADER).read()
+    for st in _lib.Status:
+        assert re.search(rf"FH_{st.name}\s*=\s*{int(st)}\b", text), st
+    for p in _lib.Precision:
+        assert re.search(rf"FH_{p.name}\s*=\s*{int(p)}\b", text), p
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(_lib.SpinComponentsC) == 24
+    assert ctypes.sizeof(_lib.StatsC) == 17 * 8
+    assert ctypes.sizeof(_lib.DomainC) == 32
+    assert ctypes.sizeof(_lib.TimingC) == 40
+
+
+def _energy(b, naux, nocc, nvir, eps, first_occ, nocc_total, kappa=float("nan"), prec=0):
+    out = _lib.SpinComponentsC()
+    bp = b.ctypes.data_as(_lib._c_dp) if b is not None else None
+    ep = eps.ctypes.data_as(_lib._c_dp)
+    st = _lib.lib().fh_rimp2_energy(bp, naux, nocc, nvir, ep, eps.size, first_occ, nocc_total,
+                                    kappa, prec, ctypes.byref(out))
+    return st, out
+
+
+def test_shape_checks_before_any_device_work():
+    L = _lib.lib()
+    eps = np.zeros(3)
+    # eps too short for the index ranges -> Layout
+    st, _ = _energy(np.zeros((4, 6)), 4, 2, 3, eps, 0, 2)
+    assert st == _lib.Status.LAYOUT
+    assert "orbital energies" in _lib.last_error()
+    # nocc * nvir overflows -> Layout
+    st, _ = _energy(np.zeros((1, 1)), 1, (1 << 64) - 1, 2, np.zeros(4), 0, 1)
+    assert st == _lib.Status.LAYOUT
+    # empty channel -> zeros, OK
+    st, out = _energy(np.zeros((4, 0)), 4, 0, 3, eps, 0, 0)
+    assert st == _lib.Status.OK and (out.e_os, out.e_ss, out.e_total) == (0.0, 0.0, 0.0)
+    st, out = _energy(np.zeros((4, 0)), 4, 3, 0, eps, 0, 3)
+    assert st == _lib.Status.OK
+    # naux == 0 -> Layout (the CPU path owns this degenerate shape)
+    st, _ = _energy(np.zeros((0, 6)), 0, 2, 3, np.zeros(5), 0, 2)
+    assert st == _lib.Status.LAYOUT
+    # kappa must be finite and > 0
+    for k in (0.0, -1.0, math.inf):
+        st, _ = _energy(np.zeros((4, 6)), 4, 2, 3, np.zeros(5), 0, 2, kappa=k)
+        assert st == _lib.Status.LAYOUT, k
+    # unknown precision
+    st, _ = _energy(np.zeros((4, 6)), 4, 2, 3, np.zeros(5), 0, 2, prec=9)
+    assert st == _lib.Status.LAYOUT
+    s = _lib.stats()
+    assert s["fail_layout"] >= 6
+    assert L.fh_reset_stats() == 0
+    assert _lib.stats()["calls"] == 0
+
+
+def test_python_errors_are_typed():
+    from ferric_amd import rimp2
+
+    with pytest.raises(_lib.LayoutError):
+        rimp2.spin_components(np.zeros((4, 6)), np.zeros(3), 2, 3)
+    with pytest.raises(_lib.LayoutError):
+        rimp2.spin_components(np.zeros((4, 7)), np.zeros(5), 2, 3)
+    sc = rimp2.spin_components(np.zeros((4, 0)), np.zeros(3), 0, 3)
+    assert sc.e_total == 0.0
+
+
+def test_try_on_device_returns_none_on_failure(monkeypatch):
+    from ferric_amd import rimp2
+
+    assert rimp2.try_spin_components_on_device(np.zeros((4, 6)), np.zeros(3), 2, 3, 0, 2) is None
+    monkeypatch.setenv("FERRIC_GPU", "off")
+    assert rimp2.try_spin_components_on_device(np.zeros((4, 0)), np.zeros(3), 0, 3, 0, 0) is None

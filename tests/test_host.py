@@ -1,0 +1,79 @@
+"""Host-side logic that needs no GPU: option parsing, SCS, generators, FP8 format."""
+import math
+
+import numpy as np
+import pytest
+
+from ferric_amd import rimp2, synthetic
+from ferric_amd._lib import LayoutError, Precision
+
+
+def test_active_occ():
+    assert rimp2.active_occ(5, 0) == 5
+    assert rimp2.active_occ(5, 1) == 4
+    assert rimp2.active_occ(0, 0) == 0
+    with pytest.raises(LayoutError):
+        rimp2.active_occ(3, 3)
+    with pytest.raises(LayoutError):
+        rimp2.active_occ(3, 7)
+
+
+def test_settings_precision(monkeypatch):
+    monkeypatch.delenv("FERRIC_GPU_PRECISION", raising=False)
+    assert rimp2.settings_precision() == Precision.FP64
+    for v, p in (("mixed", Precision.FP32), ("BF16X3", Precision.BF16X3),
+                 ("fp8", Precision.FP8_E4M3), ("f64", Precision.FP64)):
+        monkeypatch.setenv("FERRIC_GPU_PRECISION", v)
+        assert rimp2.settings_precision() == p
+    monkeypatch.setenv("FERRIC_GPU_PRECISION", "half")
+    with pytest.raises(LayoutError):
+        rimp2.settings_precision()
+
+
+def test_scs():
+    sc = rimp2.SpinComponents(-1.0, -0.5, -1.5)
+    assert rimp2.scs_mp2(sc) == pytest.approx(6 / 5 * -1.0 + 1 / 3 * -0.5)
+    assert rimp2.scs_mp2(sc, rimp2.ScsMp2Config(1.0, 1.0)) == -1.5
+    with pytest.raises(LayoutError):
+        rimp2.scs_mp2(sc, rimp2.ScsMp2Config(math.nan, 1.0))
+
+
+def test_water_shapes_distribution():
+    p = synthetic.water_shapes()
+    assert p.b_ov.shape == (84, 5 * 19) and p.b_ov.dtype == np.float64
+    eo, ev = p.eps_occ, p.eps_vir
+    assert -20.6 <= eo[0] <= -20.4 and np.all((eo[1:] >= -1.4) & (eo[1:] <= -0.45))
+    assert np.all(np.diff(eo) >= 0) and np.all(np.diff(ev) >= 0)
+    assert np.all((ev >= 0.15) & (ev <= 4.0))
+    assert abs(p.b_ov.std() - math.sqrt(0.09 / 84)) < 0.1 * math.sqrt(0.09 / 84)
+    assert np.array_equal(p.b_ov, synthetic.water_shapes().b_ov)
+
+
+def test_decane_scale_distribution():
+    p = synthetic.decane_scale()
+    assert p.b_ov.shape == (960, 41 * 209)
+    eo = p.eps_occ
+    assert np.sum(eo < -11) == 10 and np.all(eo[:10] >= -11.3) and np.all(eo[10:] <= -0.35)
+    assert np.all((p.eps_vir >= 0.12) & (p.eps_vir <= 5.0))
+    assert p.flops == pytest.approx(41 * 42 * 209 ** 2 * 960)
+
+
+def test_mixed_large_hot_rows():
+    p = synthetic.mixed_large(nocc=6, nvir=40, naux=400)
+    assert p.b_ov.dtype == np.float32
+    rs = np.sqrt(np.mean(p.b_ov.astype(np.float64) ** 2, axis=1))
+    base = math.sqrt(0.09 / 400)
+    hot = rs > 5 * base
+    assert hot.sum() == 4  # 1% of 400 rows
+    assert np.allclose(rs[hot] / base, 20, rtol=0.15)
+
+
+def test_batched_domains_shapes():
+    d = synthetic.batched_domains(64)
+    assert len(d) == 64
+    assert d.nocc.min() >= 2 and d.nocc.max() <= 12
+    assert d.nvir.min() >= 20 and d.nvir.max() <= 240
+    assert d.naux.min() >= 80 and d.naux.max() <= 720
+    p = d.domain(5)
+    assert p.b_ov.shape == (d.naux[5], d.nocc[5] * d.nvir[5])
+    assert d.flops == pytest.approx(sum(d.domain(k).flops for k in range(64)))
