@@ -1,4 +1,5 @@
-// Fused RI-MP2 pair-energy kernels for gfx950 (CDNA4 MFMA).
+// Fused RI-MP2 pair kernels for gfx950 (CDNA4 MFMA): the energy, and phase A
+// of the adjoint.
 //
 // Replaces the per-i tail GEMM G_i = B_i^T B_tail plus the host pair loop of
 // the reference (crates/ferric-mp2/src/rimp2.rs:1399-1544; device twin
@@ -18,6 +19,12 @@
 // the reference's fac = 1 (i == j) or 2 (i < j). Every term is FP64 with one
 // divide; lane sums are compensated and every reduction runs in a fixed
 // order, so the result does not depend on scheduling.
+//
+// Adjoint epilogue (same contraction): with t = g/D and
+// T~ = (c_os + c_ss) t - c_ss t^T (rimp2.rs:1987-2011), it writes the T~ block
+// of the pair for the phase-B/C GEMMs (adjoint.hip) and the per-item sums of
+// t*T~ by virtual row, virtual column and in total, from which dE/deps
+// follows (dE/dD = -t T~).
 //
 // Operands come from the packed K-major layout of pack.hip:
 // row (i*nvir_pad + a) holds B^Q_ia for Q = 0..naux_pad-1.
@@ -59,7 +66,12 @@ __device__ __forceinline__ Item decode_item(const EnergyArgs &a) {
     it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
     const int64_t pair = it.id / a.ntp;
     const int tp = (int)(it.id % a.ntp);
-    decode_pair(pair, a.nocc, it.i, it.j);
+    if (a.i_fixed >= 0) {
+        it.i = a.i_fixed;
+        it.j = a.i_fixed + (int)pair;
+    } else {
+        decode_pair(pair, a.nocc, it.i, it.j);
+    }
     decode_tile_pair(tp, a.ntile, it.ta, it.tb);
     return it;
 }
@@ -95,32 +107,199 @@ __device__ __forceinline__ void block_store(KSum os, KSum ss, double w, double *
     }
 }
 
-// The FP64 epilogue of one (a, b): both orientations of the pair loop.
-__device__ __forceinline__ void pair_terms(double g1, double g2, double d, bool kap, double kappa,
-                                           KSum &os, KSum &ss) {
-    const double inv = 1.0 / d;
-    const double dd = g1 - g2;
-    double t_os = (g1 * g1 + g2 * g2) * inv;
-    double t_ss = (dd * dd) * inv;
-    if (kap) {
-        const double d1 = 1.0 - exp(kappa * d);
-        const double damp = d1 * d1;
-        t_os *= damp;
-        t_ss *= damp;
-    }
-    os.add(t_os);
-    ss.add(t_ss);
+__device__ __forceinline__ double damping(bool kap, double kappa, double d) {
+    if (!kap) return 1.0;
+    const double d1 = 1.0 - exp(kappa * d);
+    return d1 * d1;
 }
+
+// The lane's coordinates inside the tile pair.
+struct Lane {
+    int A0, B0, wr, wc, r, g;
+};
+
+// Energy epilogue: acc1/acc2 hold g_ab / g_ba; row_of maps (g, v) to the row
+// in a 16x16 block; corr(i, a) is added to K_ii[a, a] (0 except for BF16x3).
+template <int TILE, typename Acc, typename RowOfT, typename Corr>
+__device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item &it, const Lane &L,
+                                                Acc (&acc1)[TILE / 32][TILE / 32],
+                                                Acc (&acc2)[TILE / 32][TILE / 32],
+                                                const bool *rv, const bool *cv, RowOfT row_of,
+                                                Corr corr) {
+    constexpr int TW = TILE / 2, NB = TW / 16;
+    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
+    const bool kap = !isnan(a.kappa);
+    KSum os, ss;
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            if (!(rv[x] && cv[y])) continue;
+            const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
+            if (bb >= a.nvir) continue;
+            const double eb = a.eps_vir[bb];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
+                if (aa >= a.nvir) continue;
+                const double d = eij - a.eps_vir[aa] - eb;
+                double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
+                if (it.i == it.j && aa == bb) {
+                    const double c = corr(it.i, aa);
+                    g1 += c;
+                    g2 += c;
+                }
+                const double inv = 1.0 / d;
+                const double dd = g1 - g2;
+                const double dmp = damping(kap, a.kappa, d);
+                os.add(dmp * ((g1 * g1 + g2 * g2) * inv));
+                ss.add(dmp * ((dd * dd) * inv));
+            }
+        }
+    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
+    block_store(os, ss, w, a.item_partials + 4 * it.id);
+}
+
+__device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double v) {
+    if (f32)
+        ((float *)buf)[idx] = (float)v;
+    else
+        ((double *)buf)[idx] = v;
+}
+
+// Adjoint epilogue (phase A): energy partials as above, T~ written to the
+// row's scratch blocks, and per-item sums of t*T~ (by row a, by column b,
+// total) for dE/deps. Padding elements (nvir <= a, b < nvir_pad) of T~ are
+// written as zeros: the phase-B/C GEMMs contract over them.
+template <int TILE, typename Acc, typename RowOfT, typename Corr>
+__device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item &it,
+                                                 const Lane &L,
+                                                 Acc (&acc1)[TILE / 32][TILE / 32],
+                                                 Acc (&acc2)[TILE / 32][TILE / 32],
+                                                 const bool *rv, const bool *cv, RowOfT row_of,
+                                                 Corr corr) {
+    constexpr int TW = TILE / 2, NB = TW / 16;
+    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
+    const double cd = a.c_os + a.c_ss, ce = a.c_ss;
+    const bool diag_tile = it.ta == it.tb;
+    const double wt = diag_tile ? 0.5 : 1.0;
+    const int jj = it.j - it.i;
+    const int64_t ub = (int64_t)(jj - 1) * a.nvir_pad * a.nvir_pad;
+    KSum os, ss;
+    double rsum[NB][4], csum[NB];
+#pragma unroll
+    for (int x = 0; x < NB; ++x) {
+        csum[x] = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rsum[x][v] = 0.0;
+    }
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            if (!(rv[x] && cv[y])) continue;
+            const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
+            const bool bok = bb < a.nvir;
+            const double eb = bok ? a.eps_vir[bb] : 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
+                double T1 = 0.0, T2 = 0.0;
+                if (bok && aa < a.nvir) {
+                    const double d = eij - a.eps_vir[aa] - eb;
+                    double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
+                    if (it.i == it.j && aa == bb) {
+                        const double c = corr(it.i, aa);
+                        g1 += c;
+                        g2 += c;
+                    }
+                    const double inv = 1.0 / d;
+                    const double dd = g1 - g2;
+                    os.add((g1 * g1 + g2 * g2) * inv);
+                    ss.add((dd * dd) * inv);
+                    const double t1 = g1 * inv, t2 = g2 * inv;
+                    T1 = cd * t1 - ce * t2;
+                    T2 = cd * t2 - ce * t1;
+                    const double val = wt * (t1 * T1 + t2 * T2);
+                    rsum[x][v] += val;
+                    csum[y] += val;
+                }
+                // T~_ij^ab and T~_ij^ba into the row-i blocks.
+                tstore(a.tbuf, a.t_f32, (int64_t)aa * a.ld_t + (int64_t)jj * a.nvir_pad + bb, T1);
+                if (!diag_tile)
+                    tstore(a.tbuf, a.t_f32, (int64_t)bb * a.ld_t + (int64_t)jj * a.nvir_pad + aa,
+                           T2);
+                if (jj > 0) {
+                    tstore(a.ubuf, a.t_f32, ub + (int64_t)bb * a.nvir_pad + aa, T1);
+                    if (!diag_tile) tstore(a.ubuf, a.t_f32, ub + (int64_t)aa * a.nvir_pad + bb, T2);
+                }
+            }
+        }
+    const double w = (it.i == it.j ? 1.0 : 2.0) * wt;
+    block_store(os, ss, w, a.item_partials + 4 * it.id);
+
+    // Row sums: lanes of one group g share rows; reduce over r (xor 1..8),
+    // then over the two column waves through LDS. Column sums: reduce over g
+    // (xor 16, 32), then over the two row waves.
+    __shared__ double rows_lds[2][TILE];
+    __shared__ double cols_lds[2][TILE];
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double s = rsum[x][v];
+            s += __shfl_xor(s, 1, kWave);
+            s += __shfl_xor(s, 2, kWave);
+            s += __shfl_xor(s, 4, kWave);
+            s += __shfl_xor(s, 8, kWave);
+            if (L.r == 0) rows_lds[L.wc][L.wr * TW + x * 16 + row_of(L.g, v)] = s;
+        }
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+        double s = csum[y];
+        s += __shfl_xor(s, 16, kWave);
+        s += __shfl_xor(s, 32, kWave);
+        if (L.g == 0) cols_lds[L.wr][L.wc * TW + y * 16 + L.r] = s;
+    }
+    // Blocks skipped above (beyond nvir_pad) leave their LDS slots unset:
+    // clear those slots first in a separate pass is unnecessary because the
+    // reduction below reads only rows/cols < nvir_pad - A0 / - B0.
+    __syncthreads();
+    double *vec = a.item_vec + it.id * (int64_t)a.vec_stride;
+    const int rmax = min(TILE, a.nvir_pad - L.A0), cmax = min(TILE, a.nvir_pad - L.B0);
+    for (int k = threadIdx.x; k < TILE; k += blockDim.x) {
+        vec[k] = k < rmax ? rows_lds[0][k] + rows_lds[1][k] : 0.0;
+        vec[TILE + k] = k < cmax ? cols_lds[0][k] + cols_lds[1][k] : 0.0;
+    }
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < rmax; ++k) s += rows_lds[0][k] + rows_lds[1][k];
+        vec[2 * TILE] = s;
+    }
+}
+
+struct NoCorr {
+    __device__ double operator()(int, int) const { return 0.0; }
+};
+struct LoSqCorr {
+    const double *p;
+    int nvir_pad;
+    __device__ double operator()(int i, int aa) const { return p[(int64_t)i * nvir_pad + aa]; }
+};
+template <typename T>
+struct RowOf {
+    __device__ int operator()(int g, int v) const { return out_row<T>(g, v); }
+};
 
 // ---------------------------------------------------------------------------
 // FP64 / FP32: v_mfma_{f64,f32}_16x16x4. Lane (r = l%16, g = l/16) supplies
 // A[row r][k] and B[k][col r] for its k values; the output element v of lane
-// l is C[out_row(g, v)][r]. Within a panel of KP Q values lane group g owns the
-// contiguous k range [g*KPL, g*KPL + KPL), read with one 16-byte LDS load;
+// l is C[out_row(g, v)][r]. Within a panel of KP Q values lane group g owns
+// the contiguous k range [g*KPL, g*KPL + KPL), read with one 16-byte LDS load;
 // A and B use the same k assignment, so each MFMA contracts matching Q.
 // ---------------------------------------------------------------------------
-template <typename T, int TILE, int KP>
-__global__ __launch_bounds__(256) void energy_k4_kernel(EnergyArgs a) {
+template <typename T, int TILE, int KP, bool ADJ>
+__global__ __launch_bounds__(256) void pair_k4_kernel(EnergyArgs a) {
     using M = Mfma4<T>;
     constexpr int TW = TILE / 2;          // wave sub-tile side (2x2 waves)
     constexpr int NB = TW / 16;           // 16x16 blocks per wave side
@@ -176,13 +355,12 @@ __global__ __launch_bounds__(256) void energy_k4_kernel(EnergyArgs a) {
     };
 
     const int lane = tid % kWave, wid = tid / kWave;
-    const int wr = wid / 2, wc = wid % 2;
-    const int r = lane % 16, g = lane / 16;
+    const Lane L{A0, B0, wid / 2, wid % 2, lane % 16, lane / 16};
     bool rv[NB], cv[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        rv[k] = A0 + wr * TW + k * 16 < a.nvir_pad;
-        cv[k] = B0 + wc * TW + k * 16 < a.nvir_pad;
+        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
+        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
     }
 
     typename M::acc acc1[NB][NB], acc2[NB][NB];
@@ -198,10 +376,10 @@ __global__ __launch_bounds__(256) void energy_k4_kernel(EnergyArgs a) {
         frag_t f0[NB], f1[NB], f2[NB], f3[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
-            f0[k] = *(const frag_t *)&lds[buf][0][wr * TW + k * 16 + r][g * KPL];
-            f2[k] = *(const frag_t *)&lds[buf][2][wr * TW + k * 16 + r][g * KPL];
-            f1[k] = *(const frag_t *)&lds[buf][1][wc * TW + k * 16 + r][g * KPL];
-            f3[k] = *(const frag_t *)&lds[buf][3][wc * TW + k * 16 + r][g * KPL];
+            f0[k] = *(const frag_t *)&lds[buf][0][L.wr * TW + k * 16 + L.r][L.g * KPL];
+            f2[k] = *(const frag_t *)&lds[buf][2][L.wr * TW + k * 16 + L.r][L.g * KPL];
+            f1[k] = *(const frag_t *)&lds[buf][1][L.wc * TW + k * 16 + L.r][L.g * KPL];
+            f3[k] = *(const frag_t *)&lds[buf][3][L.wc * TW + k * 16 + L.r][L.g * KPL];
         }
 #pragma unroll
         for (int s = 0; s < KPL; ++s)
@@ -227,30 +405,10 @@ __global__ __launch_bounds__(256) void energy_k4_kernel(EnergyArgs a) {
         __syncthreads();
     }
 
-    // FP64 epilogue.
-    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
-    const bool kap = !isnan(a.kappa);
-    KSum os, ss;
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            if (!(rv[x] && cv[y])) continue;
-            const int bb = B0 + wc * TW + y * 16 + r;
-            if (bb >= a.nvir) continue;
-            const double eb = a.eps_vir[bb];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int aa = A0 + wr * TW + x * 16 + out_row<T>(g, v);
-                if (aa < a.nvir) {
-                    const double d = eij - a.eps_vir[aa] - eb;
-                    pair_terms((double)acc1[x][y][v], (double)acc2[x][y][v], d, kap, a.kappa, os,
-                               ss);
-                }
-            }
-        }
-    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
-    block_store(os, ss, w, a.item_partials + 4 * it.id);
+    if constexpr (ADJ)
+        adjoint_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
+    else
+        energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
 }
 
 // ---------------------------------------------------------------------------
@@ -263,8 +421,8 @@ __global__ __launch_bounds__(256) void energy_k4_kernel(EnergyArgs a) {
 // exactly (FP64 row sums from pack.hip) on the diagonal elements K_ii[a, a],
 // the only ones where it is one-signed.
 // ---------------------------------------------------------------------------
-template <int TILE>
-__global__ __launch_bounds__(256) void energy_bf16x3_kernel(EnergyArgs a) {
+template <int TILE, bool ADJ>
+__global__ __launch_bounds__(256) void pair_bf16x3_kernel(EnergyArgs a) {
     constexpr int KP = 32;
     constexpr int TW = TILE / 2;
     constexpr int NB = TW / 16;
@@ -317,13 +475,12 @@ __global__ __launch_bounds__(256) void energy_bf16x3_kernel(EnergyArgs a) {
     };
 
     const int lane = tid % kWave, wid = tid / kWave;
-    const int wr = wid / 2, wc = wid % 2;
-    const int r = lane % 16, g = lane / 16;
+    const Lane L{A0, B0, wid / 2, wid % 2, lane % 16, lane / 16};
     bool rv[NB], cv[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        rv[k] = A0 + wr * TW + k * 16 < a.nvir_pad;
-        cv[k] = B0 + wc * TW + k * 16 < a.nvir_pad;
+        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
+        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
     }
     v4f acc1[NB][NB], acc2[NB][NB];
 #pragma unroll
@@ -341,15 +498,15 @@ __global__ __launch_bounds__(256) void energy_bf16x3_kernel(EnergyArgs a) {
         v8bf h0[NB], l0[NB], h2[NB], l2[NB], h1[NB], l1[NB], h3[NB], l3[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
-            const int ra = wr * TW + k * 16 + r, rb = wc * TW + k * 16 + r;
-            h0[k] = frag(buf, 0, ra, g);
-            l0[k] = frag(buf, 0, ra, 4 + g);
-            h2[k] = frag(buf, 2, ra, g);
-            l2[k] = frag(buf, 2, ra, 4 + g);
-            h1[k] = frag(buf, 1, rb, g);
-            l1[k] = frag(buf, 1, rb, 4 + g);
-            h3[k] = frag(buf, 3, rb, g);
-            l3[k] = frag(buf, 3, rb, 4 + g);
+            const int ra = L.wr * TW + k * 16 + L.r, rb = L.wc * TW + k * 16 + L.r;
+            h0[k] = frag(buf, 0, ra, L.g);
+            l0[k] = frag(buf, 0, ra, 4 + L.g);
+            h2[k] = frag(buf, 2, ra, L.g);
+            l2[k] = frag(buf, 2, ra, 4 + L.g);
+            h1[k] = frag(buf, 1, rb, L.g);
+            l1[k] = frag(buf, 1, rb, 4 + L.g);
+            h3[k] = frag(buf, 3, rb, L.g);
+            l3[k] = frag(buf, 3, rb, 4 + L.g);
         }
 #pragma unroll
         for (int x = 0; x < NB; ++x)
@@ -380,34 +537,11 @@ __global__ __launch_bounds__(256) void energy_bf16x3_kernel(EnergyArgs a) {
         __syncthreads();
     }
 
-    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
-    const bool kap = !isnan(a.kappa);
-    KSum os, ss;
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            if (!(rv[x] && cv[y])) continue;
-            const int bb = B0 + wc * TW + y * 16 + r;
-            if (bb >= a.nvir) continue;
-            const double eb = a.eps_vir[bb];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int aa = A0 + wr * TW + x * 16 + out_row<float>(g, v);
-                if (aa < a.nvir) {
-                    const double d = eij - a.eps_vir[aa] - eb;
-                    double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
-                    if (it.i == it.j && aa == bb) {
-                        const double c = a.diag_corr[(int64_t)it.i * a.nvir_pad + aa];
-                        g1 += c;
-                        g2 += c;
-                    }
-                    pair_terms(g1, g2, d, kap, a.kappa, os, ss);
-                }
-            }
-        }
-    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
-    block_store(os, ss, w, a.item_partials + 4 * it.id);
+    const LoSqCorr corr{a.diag_corr, a.nvir_pad};
+    if constexpr (ADJ)
+        adjoint_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<float>{}, corr);
+    else
+        energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<float>{}, corr);
 }
 
 // ---------------------------------------------------------------------------
@@ -455,22 +589,42 @@ hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
     hipError_t e = hipErrorInvalidValue;
     switch (a.prec) {
     case Prec::F64:
-        if (a.tile == 128) e = launch_items(energy_k4_kernel<double, 128, 8>, a, st);
-        else if (a.tile == 64) e = launch_items(energy_k4_kernel<double, 64, 8>, a, st);
-        else if (a.tile == 32) e = launch_items(energy_k4_kernel<double, 32, 8>, a, st);
+        if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 8, false>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 8, false>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_k4_kernel<double, 32, 8, false>, a, st);
         break;
     case Prec::F32:
-        if (a.tile == 128) e = launch_items(energy_k4_kernel<float, 128, 16>, a, st);
-        else if (a.tile == 64) e = launch_items(energy_k4_kernel<float, 64, 16>, a, st);
-        else if (a.tile == 32) e = launch_items(energy_k4_kernel<float, 32, 16>, a, st);
+        if (a.tile == 128) e = launch_items(pair_k4_kernel<float, 128, 16, false>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_k4_kernel<float, 64, 16, false>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_k4_kernel<float, 32, 16, false>, a, st);
         break;
     case Prec::BF16X3:
-        if (a.tile == 128) e = launch_items(energy_bf16x3_kernel<128>, a, st);
-        else if (a.tile == 64) e = launch_items(energy_bf16x3_kernel<64>, a, st);
-        else if (a.tile == 32) e = launch_items(energy_bf16x3_kernel<32>, a, st);
+        if (a.tile == 128) e = launch_items(pair_bf16x3_kernel<128, false>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_bf16x3_kernel<64, false>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_bf16x3_kernel<32, false>, a, st);
         break;
     case Prec::FP8:
         e = launch_energy_fp8(a, st);
+        break;
+    }
+    return e;
+}
+
+hipError_t launch_adjoint_pairs(const EnergyArgs &a, hipStream_t st) {
+    if (a.nitems <= 0 || a.nitems > 0x7fffffffLL || a.i_fixed < 0) return hipErrorInvalidValue;
+    hipError_t e = hipErrorInvalidValue;
+    switch (a.prec) {
+    case Prec::F64:
+        if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 8, true>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 8, true>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_k4_kernel<double, 32, 8, true>, a, st);
+        break;
+    case Prec::BF16X3:
+        if (a.tile == 128) e = launch_items(pair_bf16x3_kernel<128, true>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_bf16x3_kernel<64, true>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_bf16x3_kernel<32, true>, a, st);
+        break;
+    default:
         break;
     }
     return e;

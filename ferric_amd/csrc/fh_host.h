@@ -28,6 +28,10 @@ enum SlotId {
     kSlotBatch,      // batched: descriptors and work list
     kSlotBatchOut,   // batched: per-domain partials
     kSlotDiag,       // BF16: lo^2 row sums
+    kSlotStage2,     // adjoint: Q-major padded copy of B
+    kSlotU,          // adjoint: b-major T~ blocks of one row
+    kSlotAcc,        // adjoint: dE/deps accumulators
+    kSlotSplit,      // adjoint: split-K partial tiles of phase B
     kNumSlots
 };
 

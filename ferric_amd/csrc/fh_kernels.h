@@ -104,9 +104,41 @@ struct EnergyArgs {
     double *item_partials;    // 4 doubles per item
     double *row_partials;     // 4 doubles per i
     double *result;           // e_os, e_ss, then 2*nocc row values
+    // Adjoint (phase A of one occupied row) only:
+    int i_fixed = -1;         // >= 0: items are the pairs (i_fixed, j >= i_fixed)
+    double c_os = 1.0, c_ss = 1.0;
+    void *tbuf = nullptr;     // T~_ij^ab at tbuf[a*ld_t + (j-i)*nvir_pad + b]
+    void *ubuf = nullptr;     // T~_ij^ab at ubuf[((j-i-1)*nvir_pad + b)*nvir_pad + a], j > i
+    int64_t ld_t = 0;
+    bool t_f32 = false;       // T stored FP32 (reduced-precision adjoint) or FP64
+    double *item_vec = nullptr;  // per item: row sums, column sums, total of t*T~
+    int vec_stride = 0;
 };
 
 hipError_t launch_energy(const EnergyArgs &a, hipStream_t st);
+// Phase A of the adjoint: the pair kernel with the T~ / dE/deps epilogue.
+hipError_t launch_adjoint_pairs(const EnergyArgs &a, hipStream_t st);
+
+// Grouped NT GEMM of the adjoint's phases B and C for one occupied row i:
+//   gamma[Q, i*nvir + a]      += alpha sum_k B2[Q, i*nvir_pad + k] tbuf[a, k], k < ntail
+//   gamma[Q, (i+jj)*nvir + b] += alpha sum_a B2[Q, i*nvir_pad + a] ubuf_jj[b, a], jj >= 1
+struct GemmArgs {
+    Prec prec;                // F64 (double operands) or BF16X3 (float operands, split)
+    const void *b2;           // (m_pad, nov_pad) Q-major copy of B
+    int64_t ld_b2;            // nov_pad
+    const void *tbuf;         // (nvir_pad, ld_t)
+    int64_t ld_t;
+    const void *ubuf;         // (nocc-1-i) blocks of (nvir_pad, nvir_pad)
+    int i, nocc, nvir, nvir_pad, naux, m_pad;
+    int64_t nov;              // gamma row stride (nocc*nvir)
+    double alpha;
+    double *gamma;            // FP64 (naux, nov), accumulated into
+    int tile;                 // 64 or 128
+    int splits;               // phase-B split-K factor
+    int kchunk;               // phase-B k per split (multiple of the panel depth)
+    double *partial;          // (splits, m_pad, nvir_pad) phase-B partial tiles
+};
+hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st);
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
 // Row reductions and the ascending fold into a.result.
 hipError_t launch_energy_reduce(const EnergyArgs &a, hipStream_t st);

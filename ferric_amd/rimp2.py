@@ -32,7 +32,7 @@ __all__ = [
     "SpinComponents", "Precision", "active_occ", "spin_components_from_b_ov",
     "spin_components_from_b_ov_kappa", "spin_components_from_b_ov_kappa_f64",
     "try_spin_components_on_device", "spin_components", "ScsMp2Config", "scs_mp2",
-    "settings_precision",
+    "settings_precision", "energy_adjoint",
 ]
 
 
@@ -137,6 +137,65 @@ def spin_components(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
                            _kappa_c(kappa), int(precision), ctypes.byref(out))
     check(st, "fh_rimp2_energy")
     return SpinComponents(out.e_os, out.e_ss, out.e_total)
+
+
+def energy_adjoint(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
+                   nocc_total: int | None = None, c_os: float = 1.0, c_ss: float = 1.0,
+                   precision: Precision = Precision.FP64):
+    """E = c_os E_os + c_ss E_ss and its adjoint on the device.
+
+    Returns ``(SpinComponents, E, gamma, deps)``: ``gamma`` = dE/dB in the layout
+    of ``b_ov`` (naux, nocc*nvir) -- 4 x_ov of the reference's gradient
+    intermediates (crates/ferric-mp2/src/rimp2.rs:1832-1840, zvector.rs:651-697)
+    -- and ``deps`` = dE/deps over the full orbital-energy vector (2 diag P on the
+    active occupied and virtual blocks, 0 for frozen core). A device tensor
+    ``b_ov`` gives a device FP64 ``gamma``; a host array gives a numpy one.
+    Precision FP64 or BF16X3.
+    """
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    precision = Precision(precision)
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    deps = np.zeros(e.size)
+    out = _lib.SpinComponentsC()
+    L = lib()
+    if _is_torch_cuda(b_ov):
+        import torch
+
+        if b_ov.dtype == torch.float64:
+            dt = _lib.DType.F64
+        elif b_ov.dtype == torch.float32:
+            dt = _lib.DType.F32
+        else:
+            raise LayoutError(f"b_ov dtype {b_ov.dtype} is not float64/float32")
+        if b_ov.dim() != 2 or not b_ov.is_contiguous():
+            raise LayoutError("b_ov must be a contiguous 2-D (naux, nocc*nvir) tensor")
+        naux, width = b_ov.shape
+        if width != nocc * nvir:
+            raise LayoutError(f"b_ov is {naux}x{width} but nocc*nvir = {nocc * nvir}")
+        gamma = torch.empty((naux, width), dtype=torch.float64, device=b_ov.device)
+        stream = torch.cuda.current_stream(b_ov.device).cuda_stream
+        st = L.fh_rimp2_energy_adjoint_dev(
+            ctypes.c_void_p(b_ov.data_ptr()), int(dt), naux, nocc, nvir,
+            e.ctypes.data_as(_lib._c_dp), e.size, first_occ, nocc_total, float(c_os),
+            float(c_ss), int(precision), ctypes.c_void_p(stream), ctypes.byref(out),
+            ctypes.c_void_p(gamma.data_ptr()), deps.ctypes.data_as(_lib._c_dp))
+        check(st, "fh_rimp2_energy_adjoint_dev")
+    else:
+        b = np.ascontiguousarray(np.asarray(b_ov, dtype=np.float64))
+        if b.ndim != 2:
+            raise LayoutError("b_ov must be 2-D (naux, nocc*nvir)")
+        naux, width = b.shape
+        if width != nocc * nvir:
+            raise LayoutError(f"b_ov is {naux}x{width} but nocc*nvir = {nocc * nvir}")
+        gamma = np.zeros_like(b)
+        st = L.fh_rimp2_energy_adjoint(
+            b.ctypes.data_as(_lib._c_dp), naux, nocc, nvir, e.ctypes.data_as(_lib._c_dp),
+            e.size, first_occ, nocc_total, float(c_os), float(c_ss), int(precision),
+            ctypes.byref(out), gamma.ctypes.data_as(_lib._c_dp), deps.ctypes.data_as(_lib._c_dp))
+        check(st, "fh_rimp2_energy_adjoint")
+    sc = SpinComponents(out.e_os, out.e_ss, out.e_total)
+    return sc, c_os * sc.e_os + c_ss * sc.e_ss, gamma, deps
 
 
 def _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, mixed_ok) -> SpinComponents:
