@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--workload", default="mixed", choices=["mixed", "decane", "water"])
     ap.add_argument("--variants", default="FP64,FP32,BF16X3,FP8_E4M3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-rows", default="50:100", help="occupied rows of the CPU sample")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -137,6 +139,59 @@ def main():
         log(f"{prec.name}: {ms:.2f} ms/step (kernel {c_ms:.2f} ms, pack {p_ms:.2f}, reduce "
             f"{r_ms:.3f}) {r['tflops']:.1f} TFLOP/s  dE {r['dE_vs_fp64']:+.3e}")
 
+    # Energy + adjoint (FP64 vs BF16x3): the time of E_corr together with dE/dB, dE/deps.
+    adj = {}
+    if not args.no_adjoint:
+        from ferric_amd.rimp2 import energy_adjoint
+
+        for prec in (Precision.FP64, Precision.BF16X3):
+            energy_adjoint(b, p.eps, p.nocc, p.nvir, precision=prec)   # warm
+            torch.cuda.synchronize()
+            k = max(1, min(args.steps, 3))
+            t0 = time.perf_counter()
+            for _ in range(k):
+                sc, e_tot, gam, deps = energy_adjoint(b, p.eps, p.nocc, p.nvir, precision=prec)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / k
+            adj[prec.name] = {"ms_per_step": ms, "E": e_tot, "tflops_3FE": 3 * F / (ms * 1e-3) / 1e12,
+                              "gamma_abs_max": float(gam.abs().max())}
+            del gam
+            log(f"energy+adjoint {prec.name}: {ms:.1f} ms ({adj[prec.name]['tflops_3FE']:.1f} "
+                f"TFLOP/s on 3 F_E)")
+        adj["BF16X3"]["dE_vs_fp64"] = adj["BF16X3"]["E"] - adj["FP64"]["E"]
+        adj["speedup_bf16x3_vs_fp64"] = adj["FP64"]["ms_per_step"] / adj["BF16X3"]["ms_per_step"]
+
+    # Batched local-MP2 domains (512, one persistent launch per tile class, hipGraph).
+    batched = {}
+    if not args.no_batched:
+        from ferric_amd.batched import domain_set_energies
+
+        ds = synthetic.batched_domains(512)
+        bd = torch.from_numpy(ds.b_flat).to(dev)
+        ed = torch.from_numpy(ds.eps_flat).to(dev)
+        ref = None
+        for prec in (Precision.FP64, Precision.BF16X3):
+            domain_set_energies(ds, prec, True, bd, ed)   # capture + warm
+            torch.cuda.synchronize()
+            k = max(3, args.steps)
+            t0 = time.perf_counter()
+            for _ in range(k):
+                eo, es = domain_set_energies(ds, prec, True, bd, ed)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / k
+            e = eo + es
+            row = {"ms_per_step": ms, "domains_per_s": len(ds) / (ms * 1e-3),
+                   "tflops": ds.flops / (ms * 1e-3) / 1e12}
+            if ref is None:
+                ref = e
+            else:
+                row["max_abs_dE_per_domain"] = float(np.max(np.abs(e - ref)))
+                row["sum_abs_dE"] = float(np.sum(np.abs(e - ref)))
+            batched[prec.name] = row
+            log(f"batched 512 domains {prec.name}: {ms:.2f} ms, {row['domains_per_s']:.0f} "
+                f"domains/s, {row['tflops']:.1f} TFLOP/s")
+        batched["flops_per_step"] = ds.flops
+
     ok = [v for v in variants if v != Precision.FP64 and abs(res[v.name]["dE_vs_fp64"]) <= 1e-6]
     head = min(ok, key=lambda v: res[v.name]["ms_per_step"]) if ok else Precision.FP64
     h = res[head.name]
@@ -177,6 +232,8 @@ def main():
         "time_to_ecorr_ms": {k: v["ms_per_step"] for k, v in res.items()},
         "speedup_vs_fp64": res["FP64"]["ms_per_step"] / h["ms_per_step"],
         "variants": res,
+        "energy_adjoint": adj,
+        "batched_domains_512": batched,
         "roofline": {
             "bound": "mfma",
             "kernel": f"rimp2 energy contraction ({head.name})",

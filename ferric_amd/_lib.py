@@ -115,6 +115,8 @@ SIGNATURES = {
     "fh_release_workspace": (_c_int, []),
     "fh_get_stats": (_c_int, [ctypes.POINTER(StatsC)]),
     "fh_reset_stats": (_c_int, []),
+    "fh_tune_set": (_c_int, [_c_int, _c_sz, _c_sz, _c_int]),
+    "fh_tune_clear": (_c_int, []),
     "fh_set_timing": (_c_int, [_c_int]),
     "fh_last_timing": (_c_int, [ctypes.POINTER(TimingC)]),
     "fh_rimp2_energy": (_c_int, [_c_dp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz, _c_sz,
@@ -153,6 +155,9 @@ def lib() -> ctypes.CDLL:
             f.restype = res
             f.argtypes = args
         _lib = L
+        from . import autotune
+
+        autotune.install(autotune.load_table())
     return _lib
 
 

@@ -1,0 +1,113 @@
+"""Tile autotuner for the pair kernels.
+
+For each precision and each virtual-size bucket it times every candidate tile
+side on a synthetic set of local-MP2 pair domains of that bucket (the batched
+workload's distributions: nocc_d uniform on {2..12}, naux_d log-uniform on
+[80, 720]) and keeps the fastest. The table lives in
+ferric_amd/tuning/tiles.json and is installed into the library (fh_tune_set)
+whenever the library loads; problems outside every bucket keep the built-in
+choice.
+
+    python -m ferric_amd.autotune            # measure on the current GPU, write the table
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from ._lib import Precision
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning", "tiles.json")
+BUCKETS = [(20, 48), (48, 96), (96, 160), (160, 256)]
+CANDIDATES = (32, 64, 128)
+NAUX_MAX = 1 << 20
+
+
+def load_table(path: str = TABLE) -> list:
+    if not os.path.exists(path):
+        return []
+    with open(path) as f:
+        return json.load(f).get("entries", [])
+
+
+def install(entries) -> None:
+    from ._lib import check, lib
+
+    L = lib()
+    check(L.fh_tune_clear(), "fh_tune_clear")
+    for e in entries:
+        check(L.fh_tune_set(int(Precision[e["precision"]]), int(e["nvir_max"]),
+                            int(e.get("naux_max", NAUX_MAX)), int(e["tile"])), "fh_tune_set")
+
+
+def _bucket_set(lo: int, hi: int, ndom: int, seed: int):
+    from .synthetic import DomainSet, _b, _loguniform
+
+    rng = np.random.default_rng(seed)
+    nocc = rng.integers(2, 13, ndom)
+    nvir = rng.integers(lo, hi, ndom)
+    naux = np.rint(_loguniform(rng, 80, 720, ndom)).astype(np.int64)
+    sizes = naux * nocc * nvir
+    b_off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    e_off = np.concatenate([[0], np.cumsum(nocc + nvir)[:-1]]).astype(np.int64)
+    b = np.empty(int(sizes.sum()))
+    eps = np.empty(int((nocc + nvir).sum()))
+    for d in range(ndom):
+        b[b_off[d]:b_off[d] + sizes[d]] = _b(rng, int(naux[d]), int(nocc[d]), int(nvir[d])).ravel()
+        eps[e_off[d]:e_off[d] + nocc[d] + nvir[d]] = np.concatenate(
+            [np.sort(rng.uniform(-1.2, -0.35, nocc[d])), np.sort(_loguniform(rng, 0.12, 5.0, nvir[d]))])
+    return DomainSet(seed, nocc, nvir, naux, b_off, e_off, b, eps)
+
+
+def measure(precisions=(Precision.FP64, Precision.BF16X3), ndom: int = 128, reps: int = 3,
+            log=print) -> list:
+    import torch
+
+    from ._lib import check, lib
+    from .batched import domain_set_energies
+
+    check(lib().fh_tune_clear(), "fh_tune_clear")
+    entries = []
+    for prec in precisions:
+        for k, (lo, hi) in enumerate(BUCKETS):
+            ds = _bucket_set(lo, hi, ndom, seed=4000 + k)
+            b = torch.from_numpy(ds.b_flat).cuda()
+            e = torch.from_numpy(ds.eps_flat).cuda()
+            best = None
+            for tile in CANDIDATES:
+                os.environ["FERRIC_HIP_TILE"] = str(tile)
+                try:
+                    domain_set_energies(ds, prec, use_graph=False, b_dev=b, eps_dev=e)
+                    torch.cuda.synchronize()
+                    ts = []
+                    for _ in range(reps):
+                        t0 = time.perf_counter()
+                        domain_set_energies(ds, prec, use_graph=False, b_dev=b, eps_dev=e)
+                        ts.append(time.perf_counter() - t0)
+                finally:
+                    del os.environ["FERRIC_HIP_TILE"]
+                t = min(ts)
+                log(f"tune {prec.name} nvir [{lo},{hi}) tile {tile}: {t * 1e3:.2f} ms")
+                if best is None or t < best[1]:
+                    best = (tile, t)
+            entries.append({"precision": prec.name, "nvir_max": hi - 1, "naux_max": NAUX_MAX,
+                            "tile": best[0], "ms": round(best[1] * 1e3, 3)})
+    return entries
+
+
+def main():
+    entries = measure()
+    os.makedirs(os.path.dirname(TABLE), exist_ok=True)
+    with open(TABLE, "w") as f:
+        json.dump({"device": "MI355X (gfx950)", "entries": entries}, f, indent=1)
+    print(json.dumps(entries))
+    install(entries)
+
+
+if __name__ == "__main__":
+    main()

@@ -34,514 +34,37 @@
 
 #include "fh_device.h"
 #include "fh_kernels.h"
+#include "pair_body.h"
 
 namespace fh {
 
 namespace {
 
-template <typename T>
-struct Mfma4;
-template <>
-struct Mfma4<double> {
-    using acc = v4d;
-    __device__ __forceinline__ static acc mma(double x, double y, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
-    }
-};
-template <>
-struct Mfma4<float> {
-    using acc = v4f;
-    __device__ __forceinline__ static acc mma(float x, float y, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
-    }
-};
-
-struct Item {
-    int i, j, ta, tb;
-    int64_t id;
-};
-
-__device__ __forceinline__ Item decode_item(const EnergyArgs &a) {
-    Item it;
+__device__ __forceinline__ pair::Item decode_item(const EnergyArgs &a) {
+    pair::Item it;
     it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
-    const int64_t pair = it.id / a.ntp;
+    const int64_t p = it.id / a.ntp;
     const int tp = (int)(it.id % a.ntp);
     if (a.i_fixed >= 0) {
         it.i = a.i_fixed;
-        it.j = a.i_fixed + (int)pair;
+        it.j = a.i_fixed + (int)p;
     } else {
-        decode_pair(pair, a.nocc, it.i, it.j);
+        decode_pair(p, a.nocc, it.i, it.j);
     }
     decode_tile_pair(tp, a.ntile, it.ta, it.tb);
     return it;
 }
 
-// Weighted, compensated workgroup sum of (os, ss) -> 4 doubles at out4.
-__device__ __forceinline__ void block_store(KSum os, KSum ss, double w, double *out4) {
-    os.s *= w;
-    os.c *= w;
-    ss.s *= w;
-    ss.c *= w;
-    wave_merge(os);
-    wave_merge(ss);
-    __shared__ double red[16][4];
-    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (lane == 0) {
-        red[wid][0] = os.s;
-        red[wid][1] = os.c;
-        red[wid][2] = ss.s;
-        red[wid][3] = ss.c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        KSum o, s;
-        const int nw = blockDim.x / kWave;
-        for (int k = 0; k < nw; ++k) {
-            o.merge(red[k][0], red[k][1]);
-            s.merge(red[k][2], red[k][3]);
-        }
-        out4[0] = o.s;
-        out4[1] = o.c;
-        out4[2] = s.s;
-        out4[3] = s.c;
-    }
-}
-
-__device__ __forceinline__ double damping(bool kap, double kappa, double d) {
-    if (!kap) return 1.0;
-    const double d1 = 1.0 - exp(kappa * d);
-    return d1 * d1;
-}
-
-// The lane's coordinates inside the tile pair.
-struct Lane {
-    int A0, B0, wr, wc, r, g;
-};
-
-// Energy epilogue: acc1/acc2 hold g_ab / g_ba; row_of maps (g, v) to the row
-// in a 16x16 block; corr(i, a) is added to K_ii[a, a] (0 except for BF16x3).
-template <int TILE, typename Acc, typename RowOfT, typename Corr>
-__device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item &it, const Lane &L,
-                                                Acc (&acc1)[TILE / 32][TILE / 32],
-                                                Acc (&acc2)[TILE / 32][TILE / 32],
-                                                const bool *rv, const bool *cv, RowOfT row_of,
-                                                Corr corr) {
-    constexpr int TW = TILE / 2, NB = TW / 16;
-    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
-    const bool kap = !isnan(a.kappa);
-    KSum os, ss;
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            if (!(rv[x] && cv[y])) continue;
-            const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
-            if (bb >= a.nvir) continue;
-            const double eb = a.eps_vir[bb];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
-                if (aa >= a.nvir) continue;
-                const double d = eij - a.eps_vir[aa] - eb;
-                double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
-                if (it.i == it.j && aa == bb) {
-                    const double c = corr(it.i, aa);
-                    g1 += c;
-                    g2 += c;
-                }
-                const double inv = 1.0 / d;
-                const double dd = g1 - g2;
-                const double dmp = damping(kap, a.kappa, d);
-                os.add(dmp * ((g1 * g1 + g2 * g2) * inv));
-                ss.add(dmp * ((dd * dd) * inv));
-            }
-        }
-    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
-    block_store(os, ss, w, a.item_partials + 4 * it.id);
-}
-
-__device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double v) {
-    if (f32)
-        ((float *)buf)[idx] = (float)v;
-    else
-        ((double *)buf)[idx] = v;
-}
-
-// Adjoint epilogue (phase A): energy partials as above, T~ written to the
-// row's scratch blocks, and per-item sums of t*T~ (by row a, by column b,
-// total) for dE/deps. Padding elements (nvir <= a, b < nvir_pad) of T~ are
-// written as zeros: the phase-B/C GEMMs contract over them.
-template <int TILE, typename Acc, typename RowOfT, typename Corr>
-__device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item &it,
-                                                 const Lane &L,
-                                                 Acc (&acc1)[TILE / 32][TILE / 32],
-                                                 Acc (&acc2)[TILE / 32][TILE / 32],
-                                                 const bool *rv, const bool *cv, RowOfT row_of,
-                                                 Corr corr) {
-    constexpr int TW = TILE / 2, NB = TW / 16;
-    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
-    const double cd = a.c_os + a.c_ss, ce = a.c_ss;
-    const bool diag_tile = it.ta == it.tb;
-    const double wt = diag_tile ? 0.5 : 1.0;
-    const int jj = it.j - it.i;
-    const int64_t ub = (int64_t)(jj - 1) * a.nvir_pad * a.nvir_pad;
-    KSum os, ss;
-    double rsum[NB][4], csum[NB];
-#pragma unroll
-    for (int x = 0; x < NB; ++x) {
-        csum[x] = 0.0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) rsum[x][v] = 0.0;
-    }
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            if (!(rv[x] && cv[y])) continue;
-            const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
-            const bool bok = bb < a.nvir;
-            const double eb = bok ? a.eps_vir[bb] : 0.0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
-                double T1 = 0.0, T2 = 0.0;
-                if (bok && aa < a.nvir) {
-                    const double d = eij - a.eps_vir[aa] - eb;
-                    double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
-                    if (it.i == it.j && aa == bb) {
-                        const double c = corr(it.i, aa);
-                        g1 += c;
-                        g2 += c;
-                    }
-                    const double inv = 1.0 / d;
-                    const double dd = g1 - g2;
-                    os.add((g1 * g1 + g2 * g2) * inv);
-                    ss.add((dd * dd) * inv);
-                    const double t1 = g1 * inv, t2 = g2 * inv;
-                    T1 = cd * t1 - ce * t2;
-                    T2 = cd * t2 - ce * t1;
-                    const double val = wt * (t1 * T1 + t2 * T2);
-                    rsum[x][v] += val;
-                    csum[y] += val;
-                }
-                // T~_ij^ab and T~_ij^ba into the row-i blocks.
-                tstore(a.tbuf, a.t_f32, (int64_t)aa * a.ld_t + (int64_t)jj * a.nvir_pad + bb, T1);
-                if (!diag_tile)
-                    tstore(a.tbuf, a.t_f32, (int64_t)bb * a.ld_t + (int64_t)jj * a.nvir_pad + aa,
-                           T2);
-                if (jj > 0) {
-                    tstore(a.ubuf, a.t_f32, ub + (int64_t)bb * a.nvir_pad + aa, T1);
-                    if (!diag_tile) tstore(a.ubuf, a.t_f32, ub + (int64_t)aa * a.nvir_pad + bb, T2);
-                }
-            }
-        }
-    const double w = (it.i == it.j ? 1.0 : 2.0) * wt;
-    block_store(os, ss, w, a.item_partials + 4 * it.id);
-
-    // Row sums: lanes of one group g share rows; reduce over r (xor 1..8),
-    // then over the two column waves through LDS. Column sums: reduce over g
-    // (xor 16, 32), then over the two row waves.
-    __shared__ double rows_lds[2][TILE];
-    __shared__ double cols_lds[2][TILE];
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            double s = rsum[x][v];
-            s += __shfl_xor(s, 1, kWave);
-            s += __shfl_xor(s, 2, kWave);
-            s += __shfl_xor(s, 4, kWave);
-            s += __shfl_xor(s, 8, kWave);
-            if (L.r == 0) rows_lds[L.wc][L.wr * TW + x * 16 + row_of(L.g, v)] = s;
-        }
-#pragma unroll
-    for (int y = 0; y < NB; ++y) {
-        double s = csum[y];
-        s += __shfl_xor(s, 16, kWave);
-        s += __shfl_xor(s, 32, kWave);
-        if (L.g == 0) cols_lds[L.wr][L.wc * TW + y * 16 + L.r] = s;
-    }
-    // Blocks skipped above (beyond nvir_pad) leave their LDS slots unset:
-    // clear those slots first in a separate pass is unnecessary because the
-    // reduction below reads only rows/cols < nvir_pad - A0 / - B0.
-    __syncthreads();
-    double *vec = a.item_vec + it.id * (int64_t)a.vec_stride;
-    const int rmax = min(TILE, a.nvir_pad - L.A0), cmax = min(TILE, a.nvir_pad - L.B0);
-    for (int k = threadIdx.x; k < TILE; k += blockDim.x) {
-        vec[k] = k < rmax ? rows_lds[0][k] + rows_lds[1][k] : 0.0;
-        vec[TILE + k] = k < cmax ? cols_lds[0][k] + cols_lds[1][k] : 0.0;
-    }
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int k = 0; k < rmax; ++k) s += rows_lds[0][k] + rows_lds[1][k];
-        vec[2 * TILE] = s;
-    }
-}
-
-struct NoCorr {
-    __device__ double operator()(int, int) const { return 0.0; }
-};
-struct LoSqCorr {
-    const double *p;
-    int nvir_pad;
-    __device__ double operator()(int i, int aa) const { return p[(int64_t)i * nvir_pad + aa]; }
-};
-template <typename T>
-struct RowOf {
-    __device__ int operator()(int g, int v) const { return out_row<T>(g, v); }
-};
-
-// ---------------------------------------------------------------------------
-// FP64 / FP32: v_mfma_{f64,f32}_16x16x4. Lane (r = l%16, g = l/16) supplies
-// A[row r][k] and B[k][col r] for its k values; the output element v of lane
-// l is C[out_row(g, v)][r]. Within a panel of KP Q values lane group g owns
-// the contiguous k range [g*KPL, g*KPL + KPL), read with one 16-byte LDS load;
-// A and B use the same k assignment, so each MFMA contracts matching Q.
-// ---------------------------------------------------------------------------
 template <typename T, int TILE, int KP, bool ADJ>
 __global__ __launch_bounds__(256) void pair_k4_kernel(EnergyArgs a) {
-    using M = Mfma4<T>;
-    constexpr int TW = TILE / 2;          // wave sub-tile side (2x2 waves)
-    constexpr int NB = TW / 16;           // 16x16 blocks per wave side
-    constexpr int EPC = 16 / sizeof(T);   // elements per 16-byte chunk
-    constexpr int CPR = KP / EPC;         // chunks per row per panel
-    constexpr int OPCH = TILE * CPR;      // chunks per operand panel
-    constexpr int NCH = 4 * OPCH;         // chunks per panel (4 operands)
-    constexpr int CPT = (NCH + 255) / 256;
-    constexpr int LDR = KP + EPC;         // LDS row stride: one chunk of padding
-    constexpr int KPL = KP / 4;           // k values per lane per panel
-    static_assert(KPL * sizeof(T) == 16, "one 16-byte fragment load per operand block");
-    typedef T frag_t __attribute__((ext_vector_type(KPL)));
-
-    __shared__ __attribute__((aligned(16))) T lds[2][4][TILE][LDR];
-
-    const Item it = decode_item(a);
-    const T *__restrict__ bp = (const T *)a.op0;
-    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
-    // Operand rows: 0 = (i, A), 1 = (j, B), 2 = (j, A), 3 = (i, B).
-    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
-    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = (int64_t)it.i * a.nvir_pad + B0;
-    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
-    const int64_t ld = a.naux_pad;
-    const int tid = threadIdx.x;
-
-    v4i stg[CPT];
-    auto gload = [&](int q0) {
-#pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            const int c = tid + 256 * t;
-            const int op = c / OPCH, rem = c % OPCH;
-            const int row = rem / CPR, ch = rem % CPR;
-            const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
-            const int lim = (op & 1) ? limB : limA;
-            v4i v = {0, 0, 0, 0};
-            if (c < NCH && row < lim)
-                v = *(const v4i *)(bp + (base + row) * ld + q0 + ch * EPC);
-            stg[t] = v;
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            const int c = tid + 256 * t;
-            if (c < NCH) {
-                const int op = c / OPCH, rem = c % OPCH;
-                const int row = rem / CPR, ch = rem % CPR;
-                *(v4i *)&lds[buf][op][row][ch * EPC] = stg[t];
-            }
-        }
-    };
-
-    const int lane = tid % kWave, wid = tid / kWave;
-    const Lane L{A0, B0, wid / 2, wid % 2, lane % 16, lane / 16};
-    bool rv[NB], cv[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
-        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
-    }
-
-    typename M::acc acc1[NB][NB], acc2[NB][NB];
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            acc1[x][y] = (typename M::acc){0, 0, 0, 0};
-            acc2[x][y] = (typename M::acc){0, 0, 0, 0};
-        }
-
-    auto compute = [&](int buf) {
-        frag_t f0[NB], f1[NB], f2[NB], f3[NB];
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            f0[k] = *(const frag_t *)&lds[buf][0][L.wr * TW + k * 16 + L.r][L.g * KPL];
-            f2[k] = *(const frag_t *)&lds[buf][2][L.wr * TW + k * 16 + L.r][L.g * KPL];
-            f1[k] = *(const frag_t *)&lds[buf][1][L.wc * TW + k * 16 + L.r][L.g * KPL];
-            f3[k] = *(const frag_t *)&lds[buf][3][L.wc * TW + k * 16 + L.r][L.g * KPL];
-        }
-#pragma unroll
-        for (int s = 0; s < KPL; ++s)
-#pragma unroll
-            for (int x = 0; x < NB; ++x)
-#pragma unroll
-                for (int y = 0; y < NB; ++y)
-                    if (rv[x] && cv[y]) {
-                        acc1[x][y] = M::mma(f0[x][s], f1[y][s], acc1[x][y]);
-                        acc2[x][y] = M::mma(f2[x][s], f3[y][s], acc2[x][y]);
-                    }
-    };
-
-    const int np = (int)(a.naux_pad / KP);
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int p = 0; p < np; ++p) {
-        const int cur = p & 1;
-        if (p + 1 < np) gload((p + 1) * KP);
-        compute(cur);
-        if (p + 1 < np) lstore(cur ^ 1);
-        __syncthreads();
-    }
-
-    if constexpr (ADJ)
-        adjoint_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
-    else
-        energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
+    __shared__ __attribute__((aligned(16))) pair::K4Lds<T, TILE, KP> lds;
+    pair::pair_k4_body<T, TILE, KP, ADJ>(a, decode_item(a), lds);
 }
 
-// ---------------------------------------------------------------------------
-// BF16 hi/lo three-pass: v_mfma_f32_16x16x32_bf16. Lane (r, g) supplies 8
-// consecutive k values [8g, 8g+8) of a 32-deep panel for A rows and B columns
-// alike. Each 128-byte LDS row holds the panel's 32 hi then 32 lo values as
-// eight 16-byte chunks, chunk c stored at slot c ^ (row & 7) so that the
-// fragment reads of 8 consecutive rows hit distinct banks.
-// C += hi*hi + hi*lo + lo*hi in FP32; the omitted lo*lo term is added back
-// exactly (FP64 row sums from pack.hip) on the diagonal elements K_ii[a, a],
-// the only ones where it is one-signed.
-// ---------------------------------------------------------------------------
 template <int TILE, bool ADJ>
 __global__ __launch_bounds__(256) void pair_bf16x3_kernel(EnergyArgs a) {
-    constexpr int KP = 32;
-    constexpr int TW = TILE / 2;
-    constexpr int NB = TW / 16;
-    constexpr int CPR = 8;               // 4 hi + 4 lo chunks per row
-    constexpr int OPCH = TILE * CPR;
-    constexpr int NCH = 4 * OPCH;
-    constexpr int CPT = (NCH + 255) / 256;
-
-    __shared__ __attribute__((aligned(16))) v4i lds[2][4][TILE][CPR];
-
-    const Item it = decode_item(a);
-    const uint16_t *__restrict__ hi = (const uint16_t *)a.op0;
-    const uint16_t *__restrict__ lo = (const uint16_t *)a.op1;
-    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
-    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
-    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = (int64_t)it.i * a.nvir_pad + B0;
-    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
-    const int64_t ld = a.naux_pad;
-    const int tid = threadIdx.x;
-
-    v4i stg[CPT];
-    auto gload = [&](int q0) {
-#pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            const int c = tid + 256 * t;
-            const int op = c / OPCH, rem = c % OPCH;
-            const int row = rem / CPR, ch = rem % CPR;
-            const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
-            const int lim = (op & 1) ? limB : limA;
-            v4i v = {0, 0, 0, 0};
-            if (c < NCH && row < lim) {
-                const uint16_t *src = (ch < 4 ? hi : lo) + (base + row) * ld + q0 + (ch & 3) * 8;
-                v = *(const v4i *)src;
-            }
-            stg[t] = v;
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            const int c = tid + 256 * t;
-            if (c < NCH) {
-                const int op = c / OPCH, rem = c % OPCH;
-                const int row = rem / CPR, ch = rem % CPR;
-                lds[buf][op][row][ch ^ (row & 7)] = stg[t];
-            }
-        }
-    };
-
-    const int lane = tid % kWave, wid = tid / kWave;
-    const Lane L{A0, B0, wid / 2, wid % 2, lane % 16, lane / 16};
-    bool rv[NB], cv[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
-        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
-    }
-    v4f acc1[NB][NB], acc2[NB][NB];
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            acc1[x][y] = (v4f){0, 0, 0, 0};
-            acc2[x][y] = (v4f){0, 0, 0, 0};
-        }
-
-    auto frag = [&](int buf, int op, int row, int chunk) -> v8bf {
-        return __builtin_bit_cast(v8bf, lds[buf][op][row][chunk ^ (row & 7)]);
-    };
-    auto compute = [&](int buf) {
-        v8bf h0[NB], l0[NB], h2[NB], l2[NB], h1[NB], l1[NB], h3[NB], l3[NB];
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            const int ra = L.wr * TW + k * 16 + L.r, rb = L.wc * TW + k * 16 + L.r;
-            h0[k] = frag(buf, 0, ra, L.g);
-            l0[k] = frag(buf, 0, ra, 4 + L.g);
-            h2[k] = frag(buf, 2, ra, L.g);
-            l2[k] = frag(buf, 2, ra, 4 + L.g);
-            h1[k] = frag(buf, 1, rb, L.g);
-            l1[k] = frag(buf, 1, rb, 4 + L.g);
-            h3[k] = frag(buf, 3, rb, L.g);
-            l3[k] = frag(buf, 3, rb, 4 + L.g);
-        }
-#pragma unroll
-        for (int x = 0; x < NB; ++x)
-#pragma unroll
-            for (int y = 0; y < NB; ++y)
-                if (rv[x] && cv[y]) {
-                    v4f c1 = acc1[x][y], c2 = acc2[x][y];
-                    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0[x], h1[y], c1, 0, 0, 0);
-                    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l2[x], h3[y], c2, 0, 0, 0);
-                    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0[x], l1[y], c1, 0, 0, 0);
-                    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h2[x], l3[y], c2, 0, 0, 0);
-                    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0[x], h1[y], c1, 0, 0, 0);
-                    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h2[x], h3[y], c2, 0, 0, 0);
-                    acc1[x][y] = c1;
-                    acc2[x][y] = c2;
-                }
-    };
-
-    const int np = (int)(a.naux_pad / KP);
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int p = 0; p < np; ++p) {
-        const int cur = p & 1;
-        if (p + 1 < np) gload((p + 1) * KP);
-        compute(cur);
-        if (p + 1 < np) lstore(cur ^ 1);
-        __syncthreads();
-    }
-
-    const LoSqCorr corr{a.diag_corr, a.nvir_pad};
-    if constexpr (ADJ)
-        adjoint_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<float>{}, corr);
-    else
-        energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<float>{}, corr);
+    __shared__ __attribute__((aligned(16))) pair::Bf16Lds<TILE> lds;
+    pair::pair_bf16x3_body<TILE, ADJ>(a, decode_item(a), lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,7 +82,7 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const double *__restri
         os.merge(ip[4 * w], ip[4 * w + 1]);
         ss.merge(ip[4 * w + 2], ip[4 * w + 3]);
     }
-    block_store(os, ss, 1.0, rows + 4 * i);
+    pair::block_store(os, ss, 1.0, rows + 4 * i);
 }
 
 __global__ void fold_rows_kernel(const double *__restrict__ rows, int nocc,
@@ -589,9 +112,9 @@ hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
     hipError_t e = hipErrorInvalidValue;
     switch (a.prec) {
     case Prec::F64:
-        if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 8, false>, a, st);
-        else if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 8, false>, a, st);
-        else if (a.tile == 32) e = launch_items(pair_k4_kernel<double, 32, 8, false>, a, st);
+        if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 16, false>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 16, false>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_k4_kernel<double, 32, 16, false>, a, st);
         break;
     case Prec::F32:
         if (a.tile == 128) e = launch_items(pair_k4_kernel<float, 128, 16, false>, a, st);

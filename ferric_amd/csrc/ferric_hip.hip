@@ -142,6 +142,8 @@ fh_status reserve(State &s, const std::vector<std::pair<int, size_t>> &req) {
     for (const auto &r : req) {
         Slot &b = s.slots[r.first];
         if (b.bytes >= r.second) continue;
+        // Captured graphs bake in workspace pointers: drop them on any move.
+        release_graphs(s);
         if (b.ptr) {
             (void)hipFree(b.ptr);
             b.ptr = nullptr;

@@ -80,7 +80,7 @@ enum class Prec { F64 = 0, F32 = 1, BF16X3 = 2, FP8 = 3 };
 // Panel depth along Q of each precision's kernel (naux_pad is a multiple).
 inline int panel_depth(Prec p) {
     switch (p) {
-    case Prec::F64: return 8;
+    case Prec::F64: return 16;
     case Prec::F32: return 16;
     case Prec::BF16X3: return 32;
     case Prec::FP8: return 128;

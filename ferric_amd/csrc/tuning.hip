@@ -47,3 +47,27 @@ void tune_clear() {
 }
 
 }  // namespace fh
+
+using namespace fh;
+
+extern "C" fh_status fh_tune_set(fh_precision precision, size_t nvir_max, size_t naux_max,
+                                 int tile) {
+    Prec p;
+    if (!to_prec(precision, &p)) return fail(FH_LAYOUT, "unknown precision %d", (int)precision);
+    if (tile != 32 && tile != 64 && tile != 128)
+        return fail(FH_LAYOUT, "tile side %d is not 32, 64 or 128", tile);
+    tune_set((int)p, nvir_max, naux_max, tile);
+    // Captured batched graphs baked in the previous tile choice.
+    State &s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    release_graphs(s);
+    return FH_OK;
+}
+
+extern "C" fh_status fh_tune_clear(void) {
+    tune_clear();
+    State &s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    release_graphs(s);
+    return FH_OK;
+}

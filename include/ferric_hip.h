@@ -157,6 +157,14 @@ fh_status fh_release_workspace(void);
 fh_status fh_get_stats(fh_stats *out);
 fh_status fh_reset_stats(void);
 
+/* Tile table (no counterpart in the reference: this library's own tuning).
+ * Calls whose (precision, nvir, naux) fall inside an installed bucket use its
+ * tile side (32, 64 or 128); the first matching entry in installation order
+ * wins; FERRIC_HIP_TILE overrides every entry. ferric_amd/autotune.py measures
+ * the table and the Python package installs it when the library loads. */
+fh_status fh_tune_set(fh_precision precision, size_t nvir_max, size_t naux_max, int tile);
+fh_status fh_tune_clear(void);
+
 /* Phase timing on/off (see fh_timing) and the last call's phase times. */
 fh_status fh_set_timing(int on);
 fh_status fh_last_timing(fh_timing *out);

@@ -77,3 +77,51 @@ def test_batched_domains_shapes():
     p = d.domain(5)
     assert p.b_ov.shape == (d.naux[5], d.nocc[5] * d.nvir[5])
     assert d.flops == pytest.approx(sum(d.domain(k).flops for k in range(64)))
+
+
+def test_e4m3_encoder_is_round_to_nearest_even():
+    """The numpy E4M3 restatement (tests/quant.py) the FP8 parity test relies on:
+    every finite code round-trips, random values land on the nearest code."""
+    from tests.quant import e4m3_decode, e4m3_encode, fp8_scale_exponents
+
+    codes = np.arange(256, dtype=np.uint8)
+    codes = codes[(codes & 0x7F) != 0x7F]
+    vals = e4m3_decode(codes)
+    assert np.array_equal(e4m3_decode(e4m3_encode(vals.astype(np.float32))), vals)
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(100_000) * rng.choice([1e-3, 1.0, 100.0], 100_000)).astype(np.float32)
+    x = np.clip(x, -447.0, 447.0)
+    pos = np.unique(np.abs(vals))
+    a = np.abs(x.astype(np.float64))
+    idx = np.searchsorted(pos, a)
+    lo = pos[np.clip(idx - 1, 0, len(pos) - 1)]
+    hi = pos[np.clip(idx, 0, len(pos) - 1)]
+    dist = np.minimum(np.abs(a - lo), np.abs(hi - a))
+    got = np.abs(e4m3_decode(e4m3_encode(x)))
+    assert np.all(np.abs(got - a) <= dist + 1e-12)
+    assert e4m3_decode(e4m3_encode(np.float32(1000.0))) == 448.0   # saturates
+    e = fp8_scale_exponents(np.array([448.0, 449.0, 1.0, 0.0]))
+    assert list(e) == [0, 1, -8, 0]
+
+
+def test_bf16_split_residual():
+    from tests.quant import bf16_split
+
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(10_000)
+    hi, lo = bf16_split(x)
+    x32 = x.astype(np.float32).astype(np.float64)
+    assert np.all(np.abs(x32 - hi) <= 2.0 ** -8 * np.abs(x32))
+    assert np.all(np.abs(x32 - hi - lo) <= 2.0 ** -16 * np.abs(x32))
+
+
+def test_autotune_table_roundtrip(tmp_path):
+    from ferric_amd import autotune
+
+    path = tmp_path / "tiles.json"
+    path.write_text('{"entries": [{"precision": "FP64", "nvir_max": 47, "tile": 32}]}')
+    entries = autotune.load_table(str(path))
+    assert entries == [{"precision": "FP64", "nvir_max": 47, "tile": 32}]
+    assert autotune.load_table(str(tmp_path / "missing.json")) == []
+    autotune.install(entries)            # accepted by the library (no device needed)
+    autotune.install(autotune.load_table())
