@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-rows", default="50:100", help="occupied rows of the CPU sample")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's "
@@ -97,6 +99,16 @@ def main():
     dev = torch.device("cuda:0")
     b = torch.from_numpy(np.ascontiguousarray(p.b_ov)).to(dev)
     torch.cuda.synchronize()
+    if args.copy_calib_mib > 0:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE counter readings
+        # give the scale between those counters and bytes moved.
+        n = args.copy_calib_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calib_mib} MiB read + {args.copy_calib_mib} MiB written")
+        del src, dst
     F = p.flops
     set_timing(True)
 

@@ -16,44 +16,11 @@
 
 #include "fh_device.h"
 #include "fh_kernels.h"
+#include "pair_body.h"
 
 namespace fh {
 
 namespace {
-
-struct ItemF8 {
-    int i, j, ta, tb;
-    int64_t id;
-};
-
-__device__ __forceinline__ void block_store_f8(KSum os, KSum ss, double w, double *out4) {
-    os.s *= w;
-    os.c *= w;
-    ss.s *= w;
-    ss.c *= w;
-    wave_merge(os);
-    wave_merge(ss);
-    __shared__ double red[16][4];
-    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (lane == 0) {
-        red[wid][0] = os.s;
-        red[wid][1] = os.c;
-        red[wid][2] = ss.s;
-        red[wid][3] = ss.c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        KSum o, s;
-        for (int k = 0; k < (int)(blockDim.x / kWave); ++k) {
-            o.merge(red[k][0], red[k][1]);
-            s.merge(red[k][2], red[k][3]);
-        }
-        out4[0] = o.s;
-        out4[1] = o.c;
-        out4[2] = s.s;
-        out4[3] = s.c;
-    }
-}
 
 template <int TILE>
 __global__ __launch_bounds__(256) void energy_fp8_kernel(EnergyArgs a) {
@@ -67,23 +34,27 @@ __global__ __launch_bounds__(256) void energy_fp8_kernel(EnergyArgs a) {
 
     __shared__ __attribute__((aligned(16))) v4i lds[2][4][TILE][CPR];
 
-    ItemF8 it;
+    pair::Item it;
     it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
     decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
     decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
 
     const uint8_t *__restrict__ bp = (const uint8_t *)a.op0;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
+    const pair::SplitQ sq(it, a.naux_pad, KP);
+    // Operand rows as in pair_body.h: 0 = (i, A), 1 = (j, B), 2 = (j, A),
+    // 3 = (i, B), or 0 and 1 one panel ahead for a single-product item.
     const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
     const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = (int64_t)it.i * a.nvir_pad + B0;
+    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
+    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
     const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
     const int64_t ld = a.naux_pad;
     const int tid = threadIdx.x;
 
     v4i stg[CPT];
     auto gload = [&](int q0) {
+        const int q23 = q0 + sq.off23;
 #pragma unroll
         for (int t = 0; t < CPT; ++t) {
             const int c = tid + 256 * t;
@@ -91,8 +62,10 @@ __global__ __launch_bounds__(256) void energy_fp8_kernel(EnergyArgs a) {
             const int row = rem / CPR, ch = rem % CPR;
             const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
             const int lim = (op & 1) ? limB : limA;
+            const int q = op < 2 ? q0 : q23;
             v4i v = {0, 0, 0, 0};
-            if (c < NCH && row < lim) v = *(const v4i *)(bp + (base + row) * ld + q0 + ch * 16);
+            if (c < NCH && row < lim && q < a.naux_pad)
+                v = *(const v4i *)(bp + (base + row) * ld + q + ch * 16);
             stg[t] = v;
         }
     };
@@ -109,13 +82,13 @@ __global__ __launch_bounds__(256) void energy_fp8_kernel(EnergyArgs a) {
     };
 
     const int lane = tid % kWave, wid = tid / kWave;
-    const int wr = wid / 2, wc = wid % 2;
-    const int r = lane % 16, g = lane / 16;
+    const pair::Lane L{A0, B0, wid / 2, wid % 2, lane % 16, lane / 16};
+    const int g = L.g;
     bool rv[NB], cv[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        rv[k] = A0 + wr * TW + k * 16 < a.nvir_pad;
-        cv[k] = B0 + wc * TW + k * 16 < a.nvir_pad;
+        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
+        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
     }
     v4f acc1[NB][NB], acc2[NB][NB];
 #pragma unroll
@@ -134,13 +107,21 @@ __global__ __launch_bounds__(256) void energy_fp8_kernel(EnergyArgs a) {
         v4i hi = lds[buf][op][row][(g + 4) ^ (row & 7)];
         return (v8i){lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     };
+    // Scales: operands 0/1 are panel p01; operands 2/3 are panel p01 with the
+    // roles of i and j exchanged, or (single product) i and j at panel p01 + 1,
+    // whose Q block index is clamped into the table where that panel lies past
+    // naux_pad (its operands are zeros there).
     auto compute = [&](int buf, int p) {
-        const int qb = 2 * p + g / 2;
-        const int si = sc_i[qb], sj = sc_j[qb];
+        const int p01 = sq.single ? 2 * p : p;
+        const int qa = 2 * p01 + g / 2;
+        const int qn = min(qa + 2, a.nqb - 1);
+        const int s1a = sc_i[qa], s1b = sc_j[qa];
+        const int s2a = sq.single ? sc_i[qn] : s1b;
+        const int s2b = sq.single ? sc_j[qn] : s1a;
         v8i f0[NB], f1[NB], f2[NB], f3[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
-            const int ra = wr * TW + k * 16 + r, rb = wc * TW + k * 16 + r;
+            const int ra = L.wr * TW + k * 16 + L.r, rb = L.wc * TW + k * 16 + L.r;
             f0[k] = frag(buf, 0, ra);
             f2[k] = frag(buf, 2, ra);
             f1[k] = frag(buf, 1, rb);
@@ -152,55 +133,29 @@ __global__ __launch_bounds__(256) void energy_fp8_kernel(EnergyArgs a) {
             for (int y = 0; y < NB; ++y)
                 if (rv[x] && cv[y]) {
                     acc1[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                        f0[x], f1[y], acc1[x][y], 0, 0, 0, si, 0, sj);
+                        f0[x], f1[y], acc1[x][y], 0, 0, 0, s1a, 0, s1b);
                     acc2[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                        f2[x], f3[y], acc2[x][y], 0, 0, 0, sj, 0, si);
+                        f2[x], f3[y], acc2[x][y], 0, 0, 0, s2a, 0, s2b);
                 }
     };
 
-    const int np = (int)(a.naux_pad / KP);
     gload(0);
     lstore(0);
     __syncthreads();
-    for (int p = 0; p < np; ++p) {
+    for (int p = 0; p < sq.iters; ++p) {
         const int cur = p & 1;
-        if (p + 1 < np) gload((p + 1) * KP);
+        if (p + 1 < sq.iters) gload((p + 1) * sq.step);
         compute(cur, p);
-        if (p + 1 < np) lstore(cur ^ 1);
+        if (p + 1 < sq.iters) lstore(cur ^ 1);
         __syncthreads();
     }
-
-    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
-    const bool kap = !isnan(a.kappa);
-    KSum os, ss;
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            if (!(rv[x] && cv[y])) continue;
-            const int bb = B0 + wc * TW + y * 16 + r;
-            if (bb >= a.nvir) continue;
-            const double eb = a.eps_vir[bb];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int aa = A0 + wr * TW + x * 16 + out_row<float>(g, v);
-                if (aa < a.nvir) {
-                    const double d = eij - a.eps_vir[aa] - eb;
-                    const double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
-                    const double inv = 1.0 / d, dd = g1 - g2;
-                    double t_os = (g1 * g1 + g2 * g2) * inv, t_ss = dd * dd * inv;
-                    if (kap) {
-                        const double d1 = 1.0 - exp(a.kappa * d);
-                        t_os *= d1 * d1;
-                        t_ss *= d1 * d1;
-                    }
-                    os.add(t_os);
-                    ss.add(t_ss);
-                }
-            }
-        }
-    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
-    block_store_f8(os, ss, w, a.item_partials + 4 * it.id);
+    if (sq.single) {
+        static_assert(sizeof(lds) >= sizeof(float) * TW * (TILE + 1), "LDS holds a C1 half-tile");
+        pair::derive_c2<TILE, float>(it, L, acc1, acc2, pair::RowOf<float>{},
+                                     (float *)&lds[0][0][0][0]);
+    }
+    pair::energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, pair::RowOf<float>{},
+                                pair::NoCorr{});
 }
 
 }  // namespace

@@ -261,6 +261,73 @@ struct RowOf {
     __device__ int operator()(int g, int v) const { return out_row<T>(g, v); }
 };
 
+// Single-product items: for i == j the operands of C2 are those of C1
+// (C2 = C1), and on a diagonal tile pair (A == B) C2[a][b] = g_ba = C1[b][a].
+// Such an item computes C1 alone, with both accumulator sets: operands 2 and 3
+// are operands 0 and 1 again, one Q panel ahead, so acc1 sums the even panels
+// and acc2 the odd ones and the loop runs half as many iterations (a panel
+// past naux_pad reads as zeros). derive_c2 then forms C1 = acc1 + acc2 and
+// C2 from it.
+struct SplitQ {
+    bool single;
+    int off23, step, iters;
+    __device__ SplitQ(const Item &it, int64_t naux_pad, int kp) {
+        const int np = (int)(naux_pad / kp);
+        single = it.i == it.j || it.ta == it.tb;
+        off23 = single ? kp : 0;
+        step = single ? 2 * kp : kp;
+        iters = single ? (np + 1) / 2 : np;
+    }
+};
+
+// After the main loop of a single-product item: acc1 += acc2 gives C1, then
+// acc2 = C1 (i == j) or C1^T. The transpose goes through LDS (cb, at least
+// TW x (TILE + 1) elements, free once the loop's last barrier has passed) in
+// two rounds of TW rows: in round h the waves with wr == h write their rows of
+// C1 and the waves with wc == h read them transposed. Blocks the loop skipped
+// (beyond nvir_pad) hold zeros and stay zeros.
+
+template <int TILE, typename T, typename Acc, typename RowOfT>
+__device__ __forceinline__ void derive_c2(const Item &it, const Lane &L,
+                                          Acc (&acc1)[TILE / 32][TILE / 32],
+                                          Acc (&acc2)[TILE / 32][TILE / 32], RowOfT row_of,
+                                          T *cb) {
+    constexpr int TW = TILE / 2, NB = TW / 16, LDC = TILE + 1;
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            acc1[x][y] += acc2[x][y];
+            acc2[x][y] = acc1[x][y];
+        }
+    if (it.i == it.j) return;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (L.wr == h) {
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        cb[(x * 16 + row_of(L.g, v)) * LDC + L.wc * TW + y * 16 + L.r] =
+                            acc1[x][y][v];
+        }
+        __syncthreads();
+        if (L.wc == h) {
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc2[x][y][v] =
+                            cb[(y * 16 + L.r) * LDC + L.wr * TW + x * 16 + row_of(L.g, v)];
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FP64 / FP32: v_mfma_{f64,f32}_16x16x4. Lane (r = l%16, g = l/16) supplies
 // A[row r][k] and B[k][col r] for its k values; the output element v of lane
@@ -290,17 +357,20 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
 
     const T *__restrict__ bp = (const T *)a.op0;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
-    // Operand rows: 0 = (i, A), 1 = (j, B), 2 = (j, A), 3 = (i, B).
+    const SplitQ sq(it, a.naux_pad, KP);
+    // Operand rows: 0 = (i, A), 1 = (j, B), 2 = (j, A), 3 = (i, B); for a
+    // single-product item 2 and 3 are 0 and 1 again, one Q panel further on.
     const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
     const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = (int64_t)it.i * a.nvir_pad + B0;
+    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
+    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
     const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
     const int64_t ld = a.naux_pad;
     const int tid = threadIdx.x;
 
     v4i stg[CPT];
     auto gload = [&](int q0) {
+        const int q23 = q0 + sq.off23;
 #pragma unroll
         for (int t = 0; t < CPT; ++t) {
             const int c = tid + 256 * t;
@@ -308,9 +378,10 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
             const int row = rem / CPR, ch = rem % CPR;
             const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
             const int lim = (op & 1) ? limB : limA;
+            const int q = op < 2 ? q0 : q23;
             v4i v = {0, 0, 0, 0};
-            if (c < NCH && row < lim)
-                v = *(const v4i *)(bp + (base + row) * ld + q0 + ch * EPC);
+            if (c < NCH && row < lim && q < a.naux_pad)
+                v = *(const v4i *)(bp + (base + row) * ld + q + ch * EPC);
             stg[t] = v;
         }
     };
@@ -382,19 +453,22 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
 #pragma unroll
     for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
 
-    const int np = (int)(a.naux_pad / KP);
     gload(0);
     lstore(0);
     __syncthreads();
-    for (int p = 0; p < np; ++p) {
+    for (int p = 0; p < sq.iters; ++p) {
         const int cur = p & 1;
-        if (p + 1 < np) gload((p + 1) * KP);
+        if (p + 1 < sq.iters) gload((p + 1) * sq.step);
         if (full)
             compute(cur, std::true_type{});
         else
             compute(cur, std::false_type{});
-        if (p + 1 < np) lstore(cur ^ 1);
+        if (p + 1 < sq.iters) lstore(cur ^ 1);
         __syncthreads();
+    }
+    if (sq.single) {
+        static_assert(sizeof(lds) >= sizeof(T) * TW * (TILE + 1), "LDS holds a C1 half-tile");
+        derive_c2<TILE, T>(it, L, acc1, acc2, RowOf<T>{}, (T *)&lds[0][0][0][0]);
     }
 
     if constexpr (ADJ)
@@ -430,16 +504,18 @@ __device__ __forceinline__ void pair_bf16x3_body(const EnergyArgs &a, const Item
     const uint16_t *__restrict__ hi = (const uint16_t *)a.op0;
     const uint16_t *__restrict__ lo = (const uint16_t *)a.op1;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
+    const SplitQ sq(it, a.naux_pad, KP);
     const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
     const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = (int64_t)it.i * a.nvir_pad + B0;
+    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
+    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
     const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
     const int64_t ld = a.naux_pad;
     const int tid = threadIdx.x;
 
     v4i stg[CPT];
     auto gload = [&](int q0) {
+        const int q23 = q0 + sq.off23;
 #pragma unroll
         for (int t = 0; t < CPT; ++t) {
             const int c = tid + 256 * t;
@@ -447,9 +523,10 @@ __device__ __forceinline__ void pair_bf16x3_body(const EnergyArgs &a, const Item
             const int row = rem / CPR, ch = rem % CPR;
             const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
             const int lim = (op & 1) ? limB : limA;
+            const int q = op < 2 ? q0 : q23;
             v4i v = {0, 0, 0, 0};
-            if (c < NCH && row < lim) {
-                const uint16_t *src = (ch < 4 ? hi : lo) + (base + row) * ld + q0 + (ch & 3) * 8;
+            if (c < NCH && row < lim && q < a.naux_pad) {
+                const uint16_t *src = (ch < 4 ? hi : lo) + (base + row) * ld + q + (ch & 3) * 8;
                 v = *(const v4i *)src;
             }
             stg[t] = v;
@@ -518,16 +595,19 @@ __device__ __forceinline__ void pair_bf16x3_body(const EnergyArgs &a, const Item
                 }
     };
 
-    const int np = (int)(a.naux_pad / KP);
     gload(0);
     lstore(0);
     __syncthreads();
-    for (int p = 0; p < np; ++p) {
+    for (int p = 0; p < sq.iters; ++p) {
         const int cur = p & 1;
-        if (p + 1 < np) gload((p + 1) * KP);
+        if (p + 1 < sq.iters) gload((p + 1) * sq.step);
         compute(cur);
-        if (p + 1 < np) lstore(cur ^ 1);
+        if (p + 1 < sq.iters) lstore(cur ^ 1);
         __syncthreads();
+    }
+    if (sq.single) {
+        static_assert(sizeof(lds) >= sizeof(float) * TW * (TILE + 1), "LDS holds a C1 half-tile");
+        derive_c2<TILE, float>(it, L, acc1, acc2, RowOf<float>{}, (float *)&lds[0][0][0][0]);
     }
 
     const LoSqCorr corr{a.diag_corr, a.nvir_pad};
