@@ -47,6 +47,35 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+PROBE_LIB = os.path.join(ROOT, "tests", "hip", "build", "libmfma_probe.so")
+# probe kind, iterations per wave for each headline precision (about 30 ms per launch)
+RATE_PROBE = {"FP64": (0, 8000), "BF16X3": (2, 30000), "FP8_E4M3": (3, 15000)}
+
+
+def mfma_sustained(prec_name):
+    """Sustained MFMA rate of this device for the precision's instruction (TFLOP/s).
+
+    Register-only loop of independent MFMAs over 2048 workgroups
+    (tests/hip/mfma_probe.hip): the rate the matrix pipes hold at the clock this
+    load runs at. None when the probe library is absent or has no kernel for it.
+    """
+    import ctypes
+
+    if prec_name not in RATE_PROBE or not os.path.exists(PROBE_LIB):
+        return None
+    L = ctypes.CDLL(PROBE_LIB)
+    L.probe_mfma_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double)]
+    kind, iters = RATE_PROBE[prec_name]
+    t = ctypes.c_double()
+    if L.probe_mfma_rate(kind, 2048, iters, ctypes.byref(t)) != 0:
+        return None
+    rate = t.value
+    if prec_name == "BF16X3":
+        rate /= 3.0   # three MFMAs per useful product
+    return rate
+
+
 def cpu_baseline(p, rows, nthreads):
     """The oracle's OpenBLAS restatement of the reference CPU path on a sample of rows."""
     import numpy as np
@@ -208,6 +237,9 @@ def main():
     head = min(ok, key=lambda v: res[v.name]["ms_per_step"]) if ok else Precision.FP64
     h = res[head.name]
     peak = PEAK_TFLOPS[head.name]
+    sustained = mfma_sustained(head.name)
+    if sustained is not None:
+        log(f"sustained {head.name} MFMA rate (register-only probe): {sustained:.1f} TFLOP/s")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -254,6 +286,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": h["kernel_tflops"] / peak,
             "traffic": None,
+            "sustained_peak_probe": sustained,
+            "frac_of_sustained": (h["kernel_tflops"] / sustained) if sustained else None,
         },
         "cpu_baseline": cpu,
     }

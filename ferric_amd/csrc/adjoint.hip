@@ -253,7 +253,12 @@ __device__ __forceinline__ void split4(v4i in, uint32_t &h01, uint32_t &h23, uin
     uint16_t h[4], l[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const float x = __builtin_bit_cast(float, in[e]);
+        // The element goes through a scalar first: with the ROCm 7.2 clang,
+        // __builtin_bit_cast applied directly to an ext_vector element reads
+        // element 0 for every e (every chunk then held four copies of its
+        // first value).
+        const int bits = in[e];
+        const float x = __builtin_bit_cast(float, bits);
         const __bf16 hb = (__bf16)x;
         const __bf16 lb = (__bf16)(x - (float)hb);
         h[e] = __builtin_bit_cast(uint16_t, hb);

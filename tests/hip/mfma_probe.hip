@@ -76,3 +76,132 @@ extern "C" int probe_mfma(int kind, const void *a, const void *b, const int *sa,
     if (dsb) hipFree(dsb);
     return 0;
 }
+
+// ---- Sustained MFMA rate ----------------------------------------------------
+// Every wave runs `iters` rounds of NACC independent MFMAs on register
+// operands (no memory traffic inside the loop), so the launch time measures the
+// rate the matrix pipes sustain on this device at its clock under that load —
+// the practical ceiling for the pair kernels, next to the datasheet peak.
+// Each thread writes one double so the accumulators stay live.
+constexpr int kRateAcc = 16;
+
+// NACC independent f64 MFMA accumulators per wave (fewer accumulators, more
+// waves per SIMD) and, with NV > 0, NV independent vector FMA chains issued in
+// the same loop (matrix and vector pipes side by side).
+template <int NACC, int NV>
+__global__ __launch_bounds__(256) void rate_f64(int iters, double *out) {
+    const double x = 1.0 + threadIdx.x * 1e-9, y = 1.0 - blockIdx.x * 1e-9;
+    v4d acc[NACC];
+    double va[NV > 0 ? NV : 1];
+    for (int k = 0; k < NACC; ++k) acc[k] = (v4d){0, 0, 0, 0};
+    for (int k = 0; k < NV; ++k) va[k] = k * 1e-3;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) {
+            acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[k], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < NV / NACC; ++m)
+                va[k * (NV / NACC) + m] = __builtin_fma(va[k * (NV / NACC) + m], x, y);
+        }
+    }
+    double s = 0;
+    for (int k = 0; k < NACC; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    for (int k = 0; k < NV; ++k) s += va[k];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+__global__ __launch_bounds__(256) void rate_bf16(int iters, double *out) {
+    v8bf x, y;
+    for (int e = 0; e < 8; ++e) {
+        x[e] = (__bf16)(1.0f + threadIdx.x * 1e-3f);
+        y[e] = (__bf16)(1.0f - e * 1e-3f);
+    }
+    v4f acc[kRateAcc];
+    for (int k = 0; k < kRateAcc; ++k) acc[k] = (v4f){0, 0, 0, 0};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < kRateAcc; ++k)
+            acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc[k], 0, 0, 0);
+    double s = 0;
+    for (int k = 0; k < kRateAcc; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+__global__ __launch_bounds__(256) void rate_f8(int iters, double *out) {
+    v8i x, y;
+    for (int e = 0; e < 8; ++e) {
+        x[e] = 0x38383838 + (int)(threadIdx.x & 3);  // E4M3 values near 1
+        y[e] = 0x38383838;
+    }
+    v4f acc[kRateAcc];
+    for (int k = 0; k < kRateAcc; ++k) acc[k] = (v4f){0, 0, 0, 0};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < kRateAcc; ++k)
+            acc[k] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(x, y, acc[k], 0, 0, 0, 127,
+                                                                        0, 127);
+    double s = 0;
+    for (int k = 0; k < kRateAcc; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// Vector FP64 FMA: NV independent chains per lane (no MFMA), the VALU path's
+// FP64 rate for comparison with the matrix path.
+constexpr int kRateVec = 24;
+__global__ __launch_bounds__(256) void rate_vfma_f64(int iters, double *out) {
+    double acc[kRateVec];
+    const double x = 1.0 + threadIdx.x * 1e-12, y = 1.0 - blockIdx.x * 1e-12;
+    for (int k = 0; k < kRateVec; ++k) acc[k] = k * 1e-3;
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < kRateVec; ++k) acc[k] = __builtin_fma(acc[k], x, y);
+    double s = 0;
+    for (int k = 0; k < kRateVec; ++k) s += acc[k];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// kind 0 = f64 16x16x4 (16 accumulators), 2 = bf16 16x16x32, 3 = scaled f8
+// 16x16x128, 4 = vector FP64 FMA (kRateVec chains per lane), 5 / 6 = f64 MFMA
+// with 8 / 4 accumulators, 7 = 8 f64 MFMA + 16 vector FMA chains, 8 = 16 f64
+// MFMA + 32 vector FMA chains. Returns the
+// FLOP rate in TFLOP/s of the second of two identical launches in *tflops.
+extern "C" int probe_mfma_rate(int kind, int blocks, int iters, double *tflops) {
+    if (blocks <= 0 || iters <= 0) return 4;
+    double *out;
+    if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(double))) return 1;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) || hipEventCreate(&e1)) return 1;
+    // FLOP per wave per unrolled step of the loop
+    const double mf64 = 2.0 * 16 * 16 * 4, vf64 = 2.0 * 64;
+    const double flop_per_step = kind == 0   ? mf64 * kRateAcc
+                                 : kind == 2 ? 2.0 * 16 * 16 * 32 * kRateAcc
+                                 : kind == 3 ? 2.0 * 16 * 16 * 128 * kRateAcc
+                                 : kind == 4 ? vf64 * kRateVec
+                                 : kind == 5 ? mf64 * 8
+                                 : kind == 6 ? mf64 * 4
+                                 : kind == 7 ? mf64 * 8 + vf64 * 16
+                                             : mf64 * 16 + vf64 * 32;
+    float ms = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+        hipEventRecord(e0, 0);
+        switch (kind) {
+        case 0: hipLaunchKernelGGL((rate_f64<16, 0>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 5: hipLaunchKernelGGL((rate_f64<8, 0>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 6: hipLaunchKernelGGL((rate_f64<4, 0>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 7: hipLaunchKernelGGL((rate_f64<8, 16>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 8: hipLaunchKernelGGL((rate_f64<16, 32>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 2: hipLaunchKernelGGL(rate_bf16, dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 3: hipLaunchKernelGGL(rate_f8, dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 4: hipLaunchKernelGGL(rate_vfma_f64, dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        default: return 4;
+        }
+        hipEventRecord(e1, 0);
+        if (hipGetLastError() != hipSuccess) return 5;
+        if (hipEventSynchronize(e1)) return 6;
+        hipEventElapsedTime(&ms, e0, e1);
+    }
+    const double flops = (double)blocks * 4 * iters * flop_per_step;
+    *tflops = flops / (ms * 1e-3) / 1e12;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(out);
+    return 0;
+}

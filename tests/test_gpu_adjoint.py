@@ -126,9 +126,6 @@ def test_adjoint_euler_identities_decane(decane, decane_adj):
     assert e == pytest.approx(1.2 * energy.e_os + energy.e_ss / 3.0, rel=1e-13)
 
 
-@pytest.mark.xfail(reason="BF16x3 Gamma differs from FP64 by up to 55 % (energy and dE/deps "
-                          "agree); the phase-A/GEMM split of the fault is being isolated",
-                   strict=False)
 def test_bf16x3_adjoint_against_fp64(decane, decane_adj):
     p = decane
     _, e64, g64, d64 = decane_adj
