@@ -48,8 +48,51 @@ def cpu_model() -> str:
 
 
 PROBE_LIB = os.path.join(ROOT, "tests", "hip", "build", "libmfma_probe.so")
-# probe kind, iterations per wave for each headline precision (about 30 ms per launch)
-RATE_PROBE = {"FP64": (0, 8000), "BF16X3": (2, 30000), "FP8_E4M3": (3, 15000)}
+# probe kind, iterations per wave for each headline precision (about 30 ms per
+# launch); FP64 uses 8 accumulators per wave (three waves per SIMD), the
+# fastest f64 MFMA issue pattern
+RATE_PROBE = {"FP64": (5, 16000), "BF16X3": (2, 30000), "FP8_E4M3": (3, 15000)}
+# every probe kind of tests/hip/mfma_probe.hip: (kind, label, iterations)
+ALL_PROBES = [
+    (0, "f64 MFMA 16x16x4, 16 acc (1 wave/SIMD)", 8000),
+    (5, "f64 MFMA 16x16x4, 8 acc (3 waves/SIMD)", 16000),
+    (6, "f64 MFMA 16x16x4, 4 acc (7 waves/SIMD)", 32000),
+    (4, "f64 vector FMA, 24 chains", 100000),
+    (7, "f64 MFMA x8 + vector FMA x16 in one wave", 16000),
+    (9, "f64 vector FMA fed from LDS, 8x12 tile (1 wave/SIMD)", 1000),
+    (10, "f64 vector FMA fed from LDS, 8x8 tile (3 waves/SIMD)", 1500),
+    (12, "f64 vector FMA from LDS, 8x8 interleaved, 1 WG/CU", 1500),
+    (13, "f64 vector FMA from LDS, 8x8 interleaved, 3 WG/CU", 1500),
+    (2, "bf16 MFMA 16x16x32", 30000),
+    (3, "fp8 scaled MFMA 16x16x128", 15000),
+]
+# rocprofv3 summaries holding the measured HBM bytes per launch of a headline kernel
+TRAFFIC_PROFILE = {
+    ("FP64", 64): os.path.join(ROOT, "profiles", "pair_energy_fp64_t64_mixed_100_900_3600_traffic.md"),
+}
+
+
+def probe_rate(kind, iters, blocks=2048):
+    import ctypes
+
+    L = ctypes.CDLL(PROBE_LIB)
+    L.probe_mfma_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double)]
+    t = ctypes.c_double()
+    if L.probe_mfma_rate(kind, blocks, iters, ctypes.byref(t)) != 0:
+        return None
+    return t.value
+
+
+def measured_traffic(prec_name, tile):
+    """HBM bytes per launch recorded in the kernel's committed rocprofv3 summary."""
+    path = TRAFFIC_PROFILE.get((prec_name, tile))
+    if not path or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def mfma_sustained(prec_name):
@@ -59,18 +102,11 @@ def mfma_sustained(prec_name):
     (tests/hip/mfma_probe.hip): the rate the matrix pipes hold at the clock this
     load runs at. None when the probe library is absent or has no kernel for it.
     """
-    import ctypes
-
     if prec_name not in RATE_PROBE or not os.path.exists(PROBE_LIB):
         return None
-    L = ctypes.CDLL(PROBE_LIB)
-    L.probe_mfma_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                  ctypes.POINTER(ctypes.c_double)]
-    kind, iters = RATE_PROBE[prec_name]
-    t = ctypes.c_double()
-    if L.probe_mfma_rate(kind, 2048, iters, ctypes.byref(t)) != 0:
+    rate = probe_rate(*RATE_PROBE[prec_name])
+    if rate is None:
         return None
-    rate = t.value
     if prec_name == "BF16X3":
         rate /= 3.0   # three MFMAs per useful product
     return rate
@@ -104,10 +140,17 @@ def main():
     ap.add_argument("--cpu-rows", default="50:100", help="occupied rows of the CPU sample")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (calibrates HBM counters)")
+    ap.add_argument("--probe-rates", action="store_true",
+                    help="only print the MFMA / vector-FMA rate probes and exit")
     args = ap.parse_args()
+    if args.probe_rates:
+        for kind, label, iters in ALL_PROBES:
+            r = probe_rate(kind, iters)
+            print(f"{label:58s} {r:8.1f} TFLOP/s" if r else f"{label:58s} failed", flush=True)
+        return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's "
-              "multi-GPU sharding is described in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); nothing in this "
+              "path runs on more than one GPU.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -167,12 +210,13 @@ def main():
             reduce_.append(t["reduce_ms"])
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / k
-        return ms, float(np.mean(contract)), float(np.mean(pack)), float(np.mean(reduce_))
+        return (ms, float(np.mean(contract)), float(np.mean(pack)), float(np.mean(reduce_)),
+                int(t["tile"]))
 
     for prec in variants:
-        ms, c_ms, p_ms, r_ms = timed(prec, args.steps)
+        ms, c_ms, p_ms, r_ms, tile = timed(prec, args.steps)
         r = res[prec.name]
-        r.update(ms_per_step=ms, contract_ms=c_ms, pack_ms=p_ms, reduce_ms=r_ms,
+        r.update(ms_per_step=ms, contract_ms=c_ms, pack_ms=p_ms, reduce_ms=r_ms, tile=tile,
                  tflops=F / (ms * 1e-3) / 1e12, kernel_tflops=F / (c_ms * 1e-3) / 1e12,
                  dE_vs_fp64=r["E"] - e_ref)
         if prec == Precision.BF16X3:
@@ -280,12 +324,12 @@ def main():
         "batched_domains_512": batched,
         "roofline": {
             "bound": "mfma",
-            "kernel": f"rimp2 energy contraction ({head.name})",
+            "kernel": f"rimp2 energy contraction ({head.name}, tile {h['tile']})",
             "achieved": h["kernel_tflops"],
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": h["kernel_tflops"] / peak,
-            "traffic": None,
+            "traffic": measured_traffic(head.name, h.get("tile")),
             "sustained_peak_probe": sustained,
             "frac_of_sustained": (h["kernel_tflops"] / sustained) if sustained else None,
         },

@@ -60,7 +60,8 @@ class StatsC(ctypes.Structure):
 class TimingC(ctypes.Structure):
     _fields_ = [("pack_ms", ctypes.c_double), ("contract_ms", ctypes.c_double),
                 ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("items", ctypes.c_uint64)]
+                ("items", ctypes.c_uint64), ("tile", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class DomainC(ctypes.Structure):
@@ -135,6 +136,7 @@ SIGNATURES = {
 }
 
 _lib = None
+ABI_VERSION = 2   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):
@@ -154,6 +156,10 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        if L.fh_abi_version() != ABI_VERSION:
+            raise LibraryNotBuilt(
+                f"{LIB_PATH} has ABI version {L.fh_abi_version()}, this binding expects "
+                f"{ABI_VERSION}: rebuild it with `python -m ferric_amd.build`")
         _lib = L
         from . import autotune
 

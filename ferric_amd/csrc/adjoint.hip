@@ -711,7 +711,7 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     st = hip_check(hipStreamSynchronize(stream), "stream sync");
     if (st != FH_OK) return st;
     s.pending.d2h_bytes += sizeof(double) * res.size();
-    finish_timing(s, (uint64_t)total_items);
+    finish_timing(s, (uint64_t)total_items, pl.tile);
     out->e_os = res[0];
     out->e_ss = res[1];
     out->e_total = res[0] + res[1];

@@ -508,7 +508,7 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
     if (st != FH_OK) return st;
     s.pending.d2h_bytes += sizeof(double) * res.size();
     s.pending.kernel_launches += 3 + pl.classes.size();
-    finish_timing(s, (uint64_t)pl.nitems);
+    finish_timing(s, (uint64_t)pl.nitems, pl.classes.size() == 1 ? pl.classes[0].first : 0);
     for (size_t k = 0; k < ndom; ++k) {
         e_os[k] = res[2 * k];
         e_ss[k] = res[2 * k + 1];

@@ -214,9 +214,15 @@ fh_status check_kappa(double kappa, bool *has) {
     return FH_OK;
 }
 
-int choose_tile(size_t nvir) {
+// Built-in tile side (the tuning table overrides it per precision and size).
+// Large virtual spaces: 64 for FP64 / FP32 / FP8, whose 64-wide kernels run two
+// workgroups (two waves per SIMD) per CU and keep the matrix pipes busier than
+// the one-workgroup 128-wide kernels (FP64 at 100/900/3600: 669 vs 729 ms);
+// 128 for BF16x3, whose 128-wide tile halves the operand traffic per MFMA
+// (193 vs 204 ms).
+int choose_tile(size_t nvir, Prec prec) {
     if (nvir <= 24) return 32;
-    if (nvir <= 80) return 64;
+    if (nvir <= 80 || prec != Prec::BF16X3) return 64;
     return 128;
 }
 
@@ -363,7 +369,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     st = hip_check(hipStreamSynchronize(stream), "stream sync");
     if (st != FH_OK) return st;
     s.pending.d2h_bytes += sizeof(double) * res.size();
-    finish_timing(s, (uint64_t)pl.nitems);
+    finish_timing(s, (uint64_t)pl.nitems, pl.tile);
     out->e_os = res[0];
     out->e_ss = res[1];
     out->e_total = res[0] + res[1];
@@ -374,7 +380,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
 
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec) {
     EnergyPlan p;
-    p.tile = choose_tile(nvir);
+    p.tile = choose_tile(nvir, prec);
     const int tuned = tuned_tile(prec, nvir, naux);
     if (tuned > 0) p.tile = tuned;
     p.nvir_pad = (int)((nvir + 15) / 16 * 16);
@@ -419,7 +425,7 @@ void mark(State &s, int k, hipStream_t st) {
     (void)hipEventRecord(s.ev[k], st);
 }
 
-void finish_timing(State &s, uint64_t items) {
+void finish_timing(State &s, uint64_t items, int tile) {
     s.last_timing = fh_timing{};
     if (!timing_on(s)) return;
     float t[3] = {0, 0, 0}, tot = 0;
@@ -431,6 +437,7 @@ void finish_timing(State &s, uint64_t items) {
     s.last_timing.reduce_ms = t[2];
     s.last_timing.total_ms = tot;
     s.last_timing.items = items;
+    s.last_timing.tile = (uint32_t)tile;
 }
 
 bool to_prec(int p, Prec *out) {

@@ -67,7 +67,7 @@ struct State {
 // contraction), 3 (after the reductions); finish() after the stream sync.
 bool timing_on(State &s);
 void mark(State &s, int k, hipStream_t st);
-void finish_timing(State &s, uint64_t items);
+void finish_timing(State &s, uint64_t items, int tile);
 
 struct EnergyPlan {
     int tile = 128;

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 1
+#define FERRIC_HIP_ABI_VERSION 2
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -131,6 +131,8 @@ typedef struct fh_timing {
     double reduce_ms;    /* deterministic reductions */
     double total_ms;     /* first to last event */
     uint64_t items;      /* work items of the contraction launch */
+    uint32_t tile;       /* virtual tile side of the contraction kernel */
+    uint32_t reserved;
 } fh_timing;
 
 /* ---- Library state ---------------------------------------------------- */

@@ -158,10 +158,102 @@ __global__ __launch_bounds__(256) void rate_vfma_f64(int iters, double *out) {
     out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
 }
 
+// LDS-fed vector FP64 micro-kernel: a 16 x 16 thread grid, each thread an
+// MR x NR register tile; per k it reads MR values of the A panel and NR of the
+// B panel (k-major rows in LDS) and issues MR*NR FMAs. No global traffic in the
+// loop: the rate is what LDS delivery and the FP64 VALU sustain together.
+constexpr int kLdsKP = 16;
+template <int MR, int NR>
+__global__ __launch_bounds__(256) void rate_lds_fma(int iters, double *out) {
+    constexpr int TM = 16 * MR, TN = 16 * NR;
+    __shared__ __attribute__((aligned(16))) double la[kLdsKP][TM];
+    __shared__ __attribute__((aligned(16))) double lb[kLdsKP][TN];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    for (int e = tid; e < kLdsKP * TM; e += 256) (&la[0][0])[e] = 1.0 + e * 1e-9;
+    for (int e = tid; e < kLdsKP * TN; e += 256) (&lb[0][0])[e] = 1.0 - e * 1e-9;
+    __syncthreads();
+    double acc[MR][NR];
+#pragma unroll
+    for (int m = 0; m < MR; ++m)
+#pragma unroll
+        for (int n = 0; n < NR; ++n) acc[m][n] = 0.0;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll 4
+        for (int k = 0; k < kLdsKP; ++k) {
+            double a[MR], b[NR];
+#pragma unroll
+            for (int m = 0; m < MR; ++m) a[m] = la[k][ty * MR + m];
+#pragma unroll
+            for (int n = 0; n < NR; ++n) b[n] = lb[k][tx * NR + n];
+#pragma unroll
+            for (int m = 0; m < MR; ++m)
+#pragma unroll
+                for (int n = 0; n < NR; ++n) acc[m][n] = __builtin_fma(a[m], b[n], acc[m][n]);
+        }
+    }
+    double s = 0;
+#pragma unroll
+    for (int m = 0; m < MR; ++m)
+#pragma unroll
+        for (int n = 0; n < NR; ++n) s += acc[m][n];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// The same with the layout of a 128 x 128 tile: thread (tx, ty) owns rows
+// {2ty, 2ty+1} + 32 c and columns {2tx, 2tx+1} + 32 c (c = 0..3), read as
+// 16-byte chunks that sit side by side across lanes; the LDS footprint is
+// padded so one workgroup holds a CU (one wave per SIMD).
+template <int PAD_KB>
+__global__ __launch_bounds__(256) void rate_lds_fma_il(int iters, double *out) {
+    __shared__ __attribute__((aligned(16))) double la[kLdsKP][128];
+    __shared__ __attribute__((aligned(16))) double lb[kLdsKP][128];
+    __shared__ double pad[PAD_KB * 128];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    for (int e = tid; e < kLdsKP * 128; e += 256) {
+        (&la[0][0])[e] = 1.0 + e * 1e-9;
+        (&lb[0][0])[e] = 1.0 - e * 1e-9;
+    }
+    if (tid == 0) pad[0] = 0.0;
+    __syncthreads();
+    double acc[8][8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc[m][n] = 0.0;
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll 4
+        for (int k = 0; k < kLdsKP; ++k) {
+            double a[8], b[8];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const d2 va = *(const d2 *)&la[k][2 * ty + 32 * c];
+                const d2 vb = *(const d2 *)&lb[k][2 * tx + 32 * c];
+                a[2 * c] = va[0];
+                a[2 * c + 1] = va[1];
+                b[2 * c] = vb[0];
+                b[2 * c + 1] = vb[1];
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+#pragma unroll
+                for (int n = 0; n < 8; ++n) acc[m][n] = __builtin_fma(a[m], b[n], acc[m][n]);
+        }
+    }
+    double s = pad[0];
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int n = 0; n < 8; ++n) s += acc[m][n];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 // kind 0 = f64 16x16x4 (16 accumulators), 2 = bf16 16x16x32, 3 = scaled f8
 // 16x16x128, 4 = vector FP64 FMA (kRateVec chains per lane), 5 / 6 = f64 MFMA
 // with 8 / 4 accumulators, 7 = 8 f64 MFMA + 16 vector FMA chains, 8 = 16 f64
-// MFMA + 32 vector FMA chains. Returns the
+// MFMA + 32 vector FMA chains, 9 / 10 / 11 = LDS-fed vector FMA with 8x12 /
+// 8x8 / 6x8 register tiles, 12 / 13 = 8x8 interleaved layout at one / three
+// workgroups per CU. Returns the
 // FLOP rate in TFLOP/s of the second of two identical launches in *tflops.
 extern "C" int probe_mfma_rate(int kind, int blocks, int iters, double *tflops) {
     if (blocks <= 0 || iters <= 0) return 4;
@@ -178,7 +270,11 @@ extern "C" int probe_mfma_rate(int kind, int blocks, int iters, double *tflops) 
                                  : kind == 5 ? mf64 * 8
                                  : kind == 6 ? mf64 * 4
                                  : kind == 7 ? mf64 * 8 + vf64 * 16
-                                             : mf64 * 16 + vf64 * 32;
+                                 : kind == 8 ? mf64 * 16 + vf64 * 32
+                                 : kind == 9 ? vf64 * kLdsKP * 8 * 12
+                                 : kind == 10 ? vf64 * kLdsKP * 8 * 8
+                                 : kind == 11 ? vf64 * kLdsKP * 6 * 8
+                                              : vf64 * kLdsKP * 8 * 8;
     float ms = 0;
     for (int rep = 0; rep < 2; ++rep) {
         hipEventRecord(e0, 0);
@@ -188,6 +284,11 @@ extern "C" int probe_mfma_rate(int kind, int blocks, int iters, double *tflops) 
         case 6: hipLaunchKernelGGL((rate_f64<4, 0>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 7: hipLaunchKernelGGL((rate_f64<8, 16>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 8: hipLaunchKernelGGL((rate_f64<16, 32>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 9: hipLaunchKernelGGL((rate_lds_fma<8, 12>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 10: hipLaunchKernelGGL((rate_lds_fma<8, 8>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 11: hipLaunchKernelGGL((rate_lds_fma<6, 8>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 12: hipLaunchKernelGGL((rate_lds_fma_il<100>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 13: hipLaunchKernelGGL((rate_lds_fma_il<1>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 2: hipLaunchKernelGGL(rate_bf16, dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 3: hipLaunchKernelGGL(rate_f8, dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 4: hipLaunchKernelGGL(rate_vfma_f64, dim3(blocks), dim3(256), 0, 0, iters, out); break;
