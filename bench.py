@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-small-configs", action="store_true",
+                    help="skip the water-shape and decane-scale legs")
     ap.add_argument("--cpu-rows", default="50:100", help="occupied rows of the CPU sample")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (calibrates HBM counters)")
@@ -277,6 +279,66 @@ def main():
                 f"domains/s, {row['tflops']:.1f} TFLOP/s")
         batched["flops_per_step"] = ds.flops
 
+    # The other named shapes: water (5/19/84, launch-latency bound), decane scale
+    # (41/209/960) in FP64 and FP32, and the decane-scale adjoint in FP64 and BF16x3.
+    small = {}
+    if not args.no_small_configs:
+        from ferric_amd.rimp2 import energy_adjoint
+
+        def timed_ms(fn, reps):
+            ts = []
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fn()
+                torch.cuda.synchronize()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            return float(np.median(ts)), float(np.min(ts))
+
+        w = synthetic.water_shapes()
+        wb = torch.from_numpy(w.b_ov).to(dev)
+        sc = rimp2.spin_components(wb, w.eps, w.nocc, w.nvir, precision=Precision.FP64)
+        med, mn = timed_ms(lambda: rimp2.spin_components(wb, w.eps, w.nocc, w.nvir,
+                                                         precision=Precision.FP64), 50)
+        small[w.name] = {"precision": "FP64", "E_os": sc.e_os, "E_ss": sc.e_ss,
+                         "E_total": sc.e_total, "ms_median": med, "ms_min": mn}
+        log(f"{w.name}: E {sc.e_total:.12f}, {med:.3f} ms median")
+
+        dc = synthetic.decane_scale()
+        db = torch.from_numpy(dc.b_ov).to(dev)
+        row = {}
+        for prec in (Precision.FP64, Precision.FP32):
+            sc = rimp2.spin_components(db, dc.eps, dc.nocc, dc.nvir, precision=prec)
+            med, mn = timed_ms(lambda: rimp2.spin_components(db, dc.eps, dc.nocc, dc.nvir,
+                                                             precision=prec), 10)
+            row[prec.name] = {"E_os": sc.e_os, "E_ss": sc.e_ss, "E_total": sc.e_total,
+                              "ms_median": med, "ms_min": mn,
+                              "tflops": dc.flops / (med * 1e-3) / 1e12}
+        row["FP32"]["dE_vs_fp64"] = row["FP32"]["E_total"] - row["FP64"]["E_total"]
+        small[dc.name] = row
+        log(f"{dc.name}: FP64 E {row['FP64']['E_total']:.12f} ({row['FP64']['ms_median']:.2f} ms), "
+            f"FP32 dE {row['FP32']['dE_vs_fp64']:+.2e} ({row['FP32']['ms_median']:.2f} ms)")
+
+        adj_small = {}
+        outs = {}
+        for prec in (Precision.FP64, Precision.BF16X3):
+            outs[prec.name] = energy_adjoint(db, dc.eps, dc.nocc, dc.nvir, 0, dc.nocc, 1.2,
+                                             1.0 / 3.0, prec)
+            med, mn = timed_ms(lambda: energy_adjoint(db, dc.eps, dc.nocc, dc.nvir, 0, dc.nocc,
+                                                      1.2, 1.0 / 3.0, prec), 5)
+            adj_small[prec.name] = {"E_scs": outs[prec.name][1], "ms_median": med, "ms_min": mn,
+                                    "tflops_3FE": 3 * dc.flops / (med * 1e-3) / 1e12}
+        g64, d64 = outs["FP64"][2], np.asarray(outs["FP64"][3])
+        g16, d16 = outs["BF16X3"][2], np.asarray(outs["BF16X3"][3])
+        adj_small["BF16X3"]["dE_vs_fp64"] = adj_small["BF16X3"]["E_scs"] - adj_small["FP64"]["E_scs"]
+        adj_small["BF16X3"]["max_rel_dGamma"] = float((g16 - g64).abs().max() / g64.abs().max())
+        adj_small["BF16X3"]["max_rel_dEps"] = float(np.max(np.abs(d16 - d64)) / np.max(np.abs(d64)))
+        small[dc.name + " adjoint (SCS 1.2, 1/3)"] = adj_small
+        log(f"{dc.name} adjoint: FP64 {adj_small['FP64']['ms_median']:.1f} ms, BF16X3 "
+            f"{adj_small['BF16X3']['ms_median']:.1f} ms, rel dGamma "
+            f"{adj_small['BF16X3']['max_rel_dGamma']:.1e}")
+        del db, wb, outs, g64, g16
+
     ok = [v for v in variants if v != Precision.FP64 and abs(res[v.name]["dE_vs_fp64"]) <= 1e-6]
     head = min(ok, key=lambda v: res[v.name]["ms_per_step"]) if ok else Precision.FP64
     h = res[head.name]
@@ -300,6 +362,25 @@ def main():
             "time_to_ecorr_s_extrapolated": F / (rate * 1e12),
         }
         log(f"cpu baseline: {rate:.3f} TFLOP/s on {nthreads} threads ({dt:.1f} s)")
+        if small:
+            # The whole decane-scale energy on the host, min and median of 5, and its
+            # difference from the GPU FP64 result.
+            import oracle
+
+            dc = synthetic.decane_scale()
+            ts, r = [], None
+            for _ in range(5):
+                t0 = time.perf_counter()
+                r = oracle.spin_components_blas(dc.b_ov, dc.eps, dc.nocc, dc.nvir,
+                                                nthreads=nthreads)
+                ts.append((time.perf_counter() - t0) * 1e3)
+            gpu = small[dc.name]["FP64"]
+            cpu[dc.name] = {"ms_median": float(np.median(ts)), "ms_min": float(np.min(ts)),
+                            "E_os": r[0], "E_ss": r[1], "E_total": r[2],
+                            "gpu_fp64_minus_cpu_Eh": gpu["E_total"] - r[2],
+                            "gpu_fp64_speedup": float(np.median(ts)) / gpu["ms_median"]}
+            log(f"cpu {dc.name}: {np.median(ts):.1f} ms median, GPU FP64 - CPU "
+                f"{gpu['E_total'] - r[2]:+.2e} Eh")
 
     out = {
         "metric": "RI-MP2 time-to-E_corr + TFLOP/s, mixed vs FP64 on 1 GPU at |ΔE|≤1 µEh",
@@ -322,6 +403,7 @@ def main():
         "variants": res,
         "energy_adjoint": adj,
         "batched_domains_512": batched,
+        "small_configs": small,
         "roofline": {
             "bound": "mfma",
             "kernel": f"rimp2 energy contraction ({head.name}, tile {h['tile']})",
