@@ -70,6 +70,12 @@ class DomainC(ctypes.Structure):
                 ("naux", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
+class SpinChannelC(ctypes.Structure):
+    _fields_ = [("b", ctypes.c_void_p), ("nocc", ctypes.c_size_t), ("nvir", ctypes.c_size_t),
+                ("eps", ctypes.POINTER(ctypes.c_double)), ("n_eps", ctypes.c_size_t),
+                ("first_occ", ctypes.c_size_t), ("nocc_total", ctypes.c_size_t)]
+
+
 # GpuError taxonomy (crates/ferric-core/src/gpu/device.rs:16-36).
 class GpuError(RuntimeError):
     status = Status.HIP
@@ -133,10 +139,23 @@ SIGNATURES = {
                                              ctypes.POINTER(SpinComponentsC), _c_vp, _c_dp]),
     "fh_rimp2_batched_dev": (_c_int, [ctypes.POINTER(DomainC), _c_sz, _c_vp, _c_sz, _c_vp,
                                       _c_sz, _c_int, _c_int, _c_vp, _c_dp, _c_dp]),
+    "fh_u_same_spin": (_c_int, [ctypes.POINTER(SpinChannelC), _c_sz, _c_int, _c_dp]),
+    "fh_u_same_spin_dev": (_c_int, [ctypes.POINTER(SpinChannelC), _c_int, _c_sz, _c_int, _c_vp,
+                                    _c_dp]),
+    "fh_u_opposite_spin": (_c_int, [ctypes.POINTER(SpinChannelC), ctypes.POINTER(SpinChannelC),
+                                    _c_sz, _c_int, _c_dp]),
+    "fh_u_opposite_spin_dev": (_c_int, [ctypes.POINTER(SpinChannelC),
+                                        ctypes.POINTER(SpinChannelC), _c_int, _c_sz, _c_int,
+                                        _c_vp, _c_dp]),
+    "fh_rimp2_energy_signed": (_c_int, [_c_dp, _c_sz, _c_dp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz,
+                                        _c_sz, _c_sz, _c_int, ctypes.POINTER(SpinComponentsC)]),
+    "fh_rimp2_energy_signed_dev": (_c_int, [_c_vp, _c_sz, _c_vp, _c_sz, _c_int, _c_sz, _c_sz,
+                                            _c_dp, _c_sz, _c_sz, _c_sz, _c_int, _c_vp,
+                                            ctypes.POINTER(SpinComponentsC)]),
 }
 
 _lib = None
-ABI_VERSION = 2   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 3   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

@@ -61,6 +61,23 @@ __global__ __launch_bounds__(256) void pair_k4_kernel(EnergyArgs a) {
     pair::pair_k4_body<T, TILE, KP, ADJ>(a, decode_item(a), lds);
 }
 
+// Opposite-spin rectangle: item (i, J, tA, tB), J and tB on the beta side;
+// the items of alpha row i are contiguous.
+template <typename T, int TILE, int KP>
+__global__ __launch_bounds__(256) void pair_rect_kernel(EnergyArgs a) {
+    __shared__ __attribute__((aligned(16))) pair::K4Lds<T, TILE, KP> lds;
+    pair::Item it;
+    it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
+    const int64_t ntp = (int64_t)a.ntile * a.ntile_b;
+    const int64_t p = it.id / ntp;
+    const int t = (int)(it.id % ntp);
+    it.i = (int)(p / a.nocc_b);
+    it.j = (int)(p % a.nocc_b);
+    it.ta = t / a.ntile_b;
+    it.tb = t % a.ntile_b;
+    pair::pair_k4_body<T, TILE, KP, false, true>(a, it, lds);
+}
+
 template <int TILE, bool ADJ>
 __global__ __launch_bounds__(256) void pair_bf16x3_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::Bf16Lds<TILE> lds;
@@ -79,6 +96,20 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const double *__restri
     const int64_t w0 = p0 * ntp, w1 = (p0 + (nocc - i)) * ntp;
     KSum os, ss;
     for (int64_t w = w0 + threadIdx.x; w < w1; w += 256) {
+        os.merge(ip[4 * w], ip[4 * w + 1]);
+        ss.merge(ip[4 * w + 2], ip[4 * w + 3]);
+    }
+    pair::block_store(os, ss, 1.0, rows + 4 * i);
+}
+
+// Rectangle rows: items [i*per_row, (i+1)*per_row) belong to alpha row i.
+__global__ __launch_bounds__(256) void reduce_rect_rows_kernel(const double *__restrict__ ip,
+                                                               int64_t per_row,
+                                                               double *__restrict__ rows) {
+    const int i = blockIdx.x;
+    KSum os, ss;
+    for (int64_t w = (int64_t)i * per_row + threadIdx.x; w < (int64_t)(i + 1) * per_row;
+         w += 256) {
         os.merge(ip[4 * w], ip[4 * w + 1]);
         ss.merge(ip[4 * w + 2], ip[4 * w + 3]);
     }
@@ -151,6 +182,36 @@ hipError_t launch_adjoint_pairs(const EnergyArgs &a, hipStream_t st) {
         break;
     }
     return e;
+}
+
+hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st) {
+    if (a.nitems <= 0 || a.nitems > 0x7fffffffLL || a.ntile_b <= 0 || a.nocc_b <= 0)
+        return hipErrorInvalidValue;
+    hipError_t e = hipErrorInvalidValue;
+    switch (a.prec) {
+    case Prec::F64:
+        if (a.tile == 128) e = launch_items(pair_rect_kernel<double, 128, 16>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_rect_kernel<double, 64, 16>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_rect_kernel<double, 32, 16>, a, st);
+        break;
+    case Prec::F32:
+        if (a.tile == 128) e = launch_items(pair_rect_kernel<float, 128, 16>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_rect_kernel<float, 64, 16>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_rect_kernel<float, 32, 16>, a, st);
+        break;
+    default:
+        break;
+    }
+    return e;
+}
+
+hipError_t launch_rect_reduce(const EnergyArgs &a, hipStream_t st) {
+    const int64_t per_row = (int64_t)a.nocc_b * a.ntile * a.ntile_b;
+    hipLaunchKernelGGL(reduce_rect_rows_kernel, dim3((unsigned)a.nocc), dim3(256), 0, st,
+                       a.item_partials, per_row, a.row_partials);
+    hipLaunchKernelGGL(fold_rows_kernel, dim3(1), dim3(64), 0, st, a.row_partials, a.nocc,
+                       a.result);
+    return hipGetLastError();
 }
 
 hipError_t launch_energy_reduce(const EnergyArgs &a, hipStream_t st) {

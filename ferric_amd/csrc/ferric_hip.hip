@@ -265,9 +265,21 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t nau
 fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux, size_t nocc,
                       size_t nvir, const double *eps, size_t first_occ, size_t nocc_total,
                       double kappa, Prec prec, hipStream_t stream, fh_spin_components *out,
-                      double *row_partials, const EnergyPlan *forced) {
+                      double *row_partials, const EnergyPlan *forced, const void *neg_dev,
+                      size_t naux_neg) {
     const bool in_f32 = b_dtype == FH_DT_F32;
     EnergyPlan pl = forced ? *forced : plan_energy(nocc, nvir, naux, prec);
+    // A signed factor stacks B- after B+ along Q, each segment padded to the
+    // panel depth so that every panel lies in one of them.
+    const bool signed_k = neg_dev != nullptr && naux_neg > 0;
+    int q_neg = 1 << 30;
+    if (signed_k) {
+        if (prec != Prec::F64 && prec != Prec::F32)
+            return fail(FH_UNSUPPORTED, "the signed kernel runs in FP64 or FP32");
+        const size_t kp = (size_t)panel_depth(prec);
+        q_neg = (int)((naux + kp - 1) / kp * kp);
+        pl.naux_pad = q_neg + (int)((naux_neg + kp - 1) / kp * kp);
+    }
     const size_t nov = nocc * nvir;
     const size_t rows = nocc * (size_t)pl.nvir_pad;
     const size_t eb = packed_elem_bytes(prec);
@@ -290,8 +302,13 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     if (st != FH_OK) return st;
 
     if (prec != Prec::F64) {
-        st = range_check(s, b_dev, in_f32, naux * nov, naux, stream);
+        const size_t depth = naux + (signed_k ? naux_neg : 0);
+        st = range_check(s, b_dev, in_f32, naux * nov, depth, stream);
         if (st != FH_OK) return st;
+        if (signed_k) {
+            st = range_check(s, neg_dev, in_f32, naux_neg * nov, depth, stream);
+            if (st != FH_OK) return st;
+        }
     }
     std::vector<double> e(nocc + nvir);
     for (size_t i = 0; i < nocc; ++i) e[i] = eps[first_occ + i];
@@ -320,7 +337,21 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     pa.scale = prec == Prec::FP8 ? (uint8_t *)s.slots[kSlotScale].ptr : nullptr;
     pa.lo_sq = prec == Prec::BF16X3 ? (double *)s.slots[kSlotDiag].ptr : nullptr;
     mark(s, 0, stream);
-    hipError_t he = launch_pack(pa, stream);
+    hipError_t he;
+    if (signed_k) {
+        PackArgs pn = pa;
+        pa.ld = pl.naux_pad;
+        pa.naux_pad = q_neg;
+        pn.ld = pl.naux_pad;
+        pn.b = neg_dev;
+        pn.naux = (int)naux_neg;
+        pn.naux_pad = pl.naux_pad - q_neg;
+        pn.out0 = (char *)pa.out0 + (size_t)q_neg * eb;
+        he = launch_pack(pa, stream);
+        if (he == hipSuccess) he = launch_pack(pn, stream);
+    } else {
+        he = launch_pack(pa, stream);
+    }
     if (he != hipSuccess) {
         (void)hipGetLastError();
         return fail(FH_KERNEL, "pack kernel: %s", hipGetErrorString(he));
@@ -347,6 +378,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     ea.item_partials = (double *)s.slots[kSlotItems].ptr;
     ea.row_partials = (double *)s.slots[kSlotRows].ptr;
     ea.result = (double *)s.slots[kSlotResult].ptr;
+    ea.q_neg = q_neg;
     mark(s, 1, stream);
     he = launch_energy(ea, stream);
     if (he != hipSuccess) {

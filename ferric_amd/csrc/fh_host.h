@@ -93,10 +93,13 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t nau
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec);
 int tuned_tile(Prec prec, size_t nvir, size_t naux);
 size_t packed_elem_bytes(Prec p);
+// neg_dev (naux_neg rows, same layout and dtype as b_dev) makes the kernel
+// signed: K = B^T B - N^T N (FP64 / FP32 only).
 fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux, size_t nocc,
                       size_t nvir, const double *eps, size_t first_occ, size_t nocc_total,
                       double kappa, Prec prec, hipStream_t stream, fh_spin_components *out,
-                      double *row_partials, const EnergyPlan *forced);
+                      double *row_partials, const EnergyPlan *forced,
+                      const void *neg_dev = nullptr, size_t naux_neg = 0);
 void commit(State &s, Prec prec, bool adjoint, bool batched);
 bool to_prec(int p, Prec *out);
 void release_graphs(State &s);

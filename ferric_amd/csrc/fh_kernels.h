@@ -67,6 +67,10 @@ struct PackArgs {
     void *out1;       // BF16 lo
     uint8_t *scale;   // FP8: (nocc, naux_pad/64) E8M0 bytes
     double *lo_sq;    // BF16: per packed row, sum over Q of lo^2 (may be null)
+    // F64/F32 only: row stride of the packed rows when this call fills one
+    // segment of them (the two factors of a signed kernel); 0 = naux_pad. The
+    // segment is naux_pad columns wide and out0 points at its first column.
+    int64_t ld = 0;
 };
 
 hipError_t launch_pack(const PackArgs &a, hipStream_t st);
@@ -113,6 +117,14 @@ struct EnergyArgs {
     bool t_f32 = false;       // T stored FP32 (reduced-precision adjoint) or FP64
     double *item_vec = nullptr;  // per item: row sums, column sums, total of t*T~
     int vec_stride = 0;
+    // Signed factor (K = B+^T B+ - B-^T B-): packed Q columns from q_neg on
+    // hold B- and enter with -1.
+    int q_neg = 1 << 30;
+    // Opposite-spin rectangle: op0 holds the alpha rows (nocc, nvir, nvir_pad,
+    // eps_occ, eps_vir), op1 the beta rows; items are (i, J, tA, tB) over every
+    // alpha i, beta J and tile of each side.
+    int nocc_b = 0, nvir_b = 0, nvir_pad_b = 0, ntile_b = 0;
+    const double *eps_occ_b = nullptr, *eps_vir_b = nullptr;
 };
 
 hipError_t launch_energy(const EnergyArgs &a, hipStream_t st);
@@ -140,6 +152,9 @@ struct GemmArgs {
 };
 hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st);
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
+// Opposite-spin rectangle (F64 / F32) and its row reduction + fold.
+hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st);
+hipError_t launch_rect_reduce(const EnergyArgs &a, hipStream_t st);
 // Row reductions and the ascending fold into a.result.
 hipError_t launch_energy_reduce(const EnergyArgs &a, hipStream_t st);
 

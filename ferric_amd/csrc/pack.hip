@@ -9,7 +9,10 @@
 //
 // with nvir_pad = round_up(nvir, 16) and naux_pad = round_up(naux, KP) of the
 // consuming kernel; padding rows and columns are written as zeros, so no
-// memset is needed and the kernels' K loops need no bounds checks.
+// memset is needed and the kernels' K loops need no bounds checks. A signed
+// factor (B+ and B-, rimp2.rs:1559-1607) fills two column segments of the same
+// rows: the FP64 / FP32 packers then write with a row stride ld wider than the
+// segment, starting at the segment's first column.
 //
 // Formats: FP64, FP32, BF16 hi/lo split (hi = RN(x), lo = RN(x - hi)), and
 // OCP FP8 E4M3 with one power-of-two (E8M0) scale per (occupied i, 64-row Q
@@ -37,8 +40,8 @@ __device__ __forceinline__ float load_as_float(const Tin *p) {
 template <typename Tin, typename Tout, typename Conv>
 __device__ __forceinline__ void transpose_tile(const Tin *__restrict__ b, int64_t nov, int naux,
                                                int nvir, int nvir_pad, int64_t nrows,
-                                               int naux_pad, Conv conv, Tout *__restrict__ out0,
-                                               Tout *__restrict__ out1) {
+                                               int naux_pad, int64_t ld, Conv conv,
+                                               Tout *__restrict__ out0, Tout *__restrict__ out1) {
     __shared__ float tile[PT][PT + 1];
     const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;  // 64 x 4
     const int64_t r0 = (int64_t)blockIdx.x * PT;
@@ -58,11 +61,11 @@ __device__ __forceinline__ void transpose_tile(const Tin *__restrict__ b, int64_
         tile[q][tx] = v;
     }
     __syncthreads();
-    // Store: out[(r0+rr)*naux_pad + q0 + tx] = tile[tx][rr].
+    // Store: out[(r0+rr)*ld + q0 + tx] = tile[tx][rr].
     for (int rr = ty; rr < PT; rr += 4) {
         const int64_t orow = r0 + rr;
         if (orow < nrows && q0 + tx < naux_pad) {
-            const int64_t o = orow * naux_pad + q0 + tx;
+            const int64_t o = orow * ld + q0 + tx;
             conv(tile[tx][rr], out0 + o, out1 ? out1 + o : nullptr);
         }
     }
@@ -73,7 +76,7 @@ template <typename Tin>
 __global__ __launch_bounds__(256) void pack_f64_kernel(const Tin *__restrict__ b, int64_t nov,
                                                        int naux, int nvir, int nvir_pad,
                                                        int64_t nrows, int naux_pad,
-                                                       double *__restrict__ out) {
+                                                       double *__restrict__ out, int64_t ld) {
     __shared__ double tile[PT][PT + 1];
     const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;
     const int64_t r0 = (int64_t)blockIdx.x * PT;
@@ -94,7 +97,7 @@ __global__ __launch_bounds__(256) void pack_f64_kernel(const Tin *__restrict__ b
     __syncthreads();
     for (int rr = ty; rr < PT; rr += 4) {
         const int64_t orow = r0 + rr;
-        if (orow < nrows && q0 + tx < naux_pad) out[orow * naux_pad + q0 + tx] = tile[tx][rr];
+        if (orow < nrows && q0 + tx < naux_pad) out[orow * ld + q0 + tx] = tile[tx][rr];
     }
 }
 
@@ -116,8 +119,8 @@ template <typename Tin>
 __global__ __launch_bounds__(256) void pack_f32_kernel(const Tin *__restrict__ b, int64_t nov,
                                                        int naux, int nvir, int nvir_pad,
                                                        int64_t nrows, int naux_pad,
-                                                       float *__restrict__ out) {
-    transpose_tile<Tin, float>(b, nov, naux, nvir, nvir_pad, nrows, naux_pad, ConvF32{}, out,
+                                                       float *__restrict__ out, int64_t ld) {
+    transpose_tile<Tin, float>(b, nov, naux, nvir, nvir_pad, nrows, naux_pad, ld, ConvF32{}, out,
                                (float *)nullptr);
 }
 
@@ -127,8 +130,8 @@ __global__ __launch_bounds__(256) void pack_bf16x3_kernel(const Tin *__restrict_
                                                           int64_t nrows, int naux_pad,
                                                           uint16_t *__restrict__ hi,
                                                           uint16_t *__restrict__ lo) {
-    transpose_tile<Tin, uint16_t>(b, nov, naux, nvir, nvir_pad, nrows, naux_pad, ConvBf16Split{},
-                                  hi, lo);
+    transpose_tile<Tin, uint16_t>(b, nov, naux, nvir, nvir_pad, nrows, naux_pad, naux_pad,
+                                  ConvBf16Split{}, hi, lo);
 }
 
 // Per (i, 64-row Q block) maximum |B| over finite entries -> E8M0 scale
@@ -250,6 +253,9 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t st) {
     const int64_t nov = (int64_t)a.nocc * a.nvir;
     dim3 grid = pack_grid(nrows, a.naux_pad);
     const bool f32in = a.in_f32;
+    const int64_t ld = a.ld > 0 ? a.ld : a.naux_pad;
+    if (ld != a.naux_pad && a.fmt != PackFmt::F64 && a.fmt != PackFmt::F32)
+        return hipErrorInvalidValue;
 #define FH_LAUNCH(K, ...)                                                                      \
     do {                                                                                       \
         if (f32in)                                                                             \
@@ -261,10 +267,10 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t st) {
     } while (0)
     switch (a.fmt) {
     case PackFmt::F64:
-        FH_LAUNCH(pack_f64_kernel, (double *)a.out0);
+        FH_LAUNCH(pack_f64_kernel, (double *)a.out0, ld);
         break;
     case PackFmt::F32:
-        FH_LAUNCH(pack_f32_kernel, (float *)a.out0);
+        FH_LAUNCH(pack_f32_kernel, (float *)a.out0, ld);
         break;
     case PackFmt::BF16X2:
         FH_LAUNCH(pack_bf16x3_kernel, (uint16_t *)a.out0, (uint16_t *)a.out1);

@@ -271,9 +271,9 @@ struct RowOf {
 struct SplitQ {
     bool single;
     int off23, step, iters;
-    __device__ SplitQ(const Item &it, int64_t naux_pad, int kp) {
+    __device__ SplitQ(const Item &it, int64_t naux_pad, int kp, bool always = false) {
         const int np = (int)(naux_pad / kp);
-        single = it.i == it.j || it.ta == it.tb;
+        single = always || it.i == it.j || it.ta == it.tb;
         off23 = single ? kp : 0;
         step = single ? 2 * kp : kp;
         iters = single ? (np + 1) / 2 : np;
@@ -328,6 +328,36 @@ __device__ __forceinline__ void derive_c2(const Item &it, const Lane &L,
     }
 }
 
+// Opposite-spin rectangle epilogue (u_rimp2.rs:731-760): acc holds
+// (ia|JB) for a in the alpha tile A and B in the beta tile; the item adds
+// sum (ia|JB)^2 / D with D = e_i + e_J - e_a - e_B, weight 1 (every (i, J) and
+// every tile pair is its own item), into the os slot of its partial.
+template <int TILE, typename Acc, typename RowOfT>
+__device__ __forceinline__ void rect_epilogue(const EnergyArgs &a, const Item &it, const Lane &L,
+                                              Acc (&acc)[TILE / 32][TILE / 32], const bool *rv,
+                                              const bool *cv, RowOfT row_of) {
+    constexpr int TW = TILE / 2, NB = TW / 16;
+    const double eij = a.eps_occ[it.i] + a.eps_occ_b[it.j];
+    KSum os, ss;
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            if (!(rv[x] && cv[y])) continue;
+            const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
+            if (bb >= a.nvir_b) continue;
+            const double eb = a.eps_vir_b[bb];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
+                if (aa >= a.nvir) continue;
+                const double g = acc[x][y][v];
+                os.add((g * g) / (eij - a.eps_vir[aa] - eb));
+            }
+        }
+    block_store(os, ss, 1.0, a.item_partials + 4 * it.id);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 / FP32: v_mfma_{f64,f32}_16x16x4. Lane (r = l%16, g = l/16) supplies
 // A[row r][k] and B[k][col r] for its k values; the output element v of lane
@@ -338,7 +368,12 @@ __device__ __forceinline__ void derive_c2(const Item &it, const Lane &L,
 template <typename T, int TILE, int KP>
 using K4Lds = T[2][4][TILE][KP + 16 / sizeof(T)];
 
-template <typename T, int TILE, int KP, bool ADJ>
+// RECT: the opposite-spin rectangle (op0 alpha rows, op1 beta rows, it.j the
+// beta occupied index); every item is single-product and ends in
+// rect_epilogue. Panels whose first Q column lies at or past a.q_neg belong to
+// the negative factor of a signed kernel: their A-side fragments are negated,
+// so the MFMA chain accumulates B+^T B+ - B-^T B-.
+template <typename T, int TILE, int KP, bool ADJ, bool RECT = false>
 __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it,
                                              K4Lds<T, TILE, KP> &lds) {
     using M = Mfma4<T>;
@@ -355,16 +390,19 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
     typedef T frag_t __attribute__((ext_vector_type(KPL)));
     typedef T chunk_t __attribute__((ext_vector_type(EPC)));
 
-    const T *__restrict__ bp = (const T *)a.op0;
+    const T *__restrict__ bpa = (const T *)a.op0;
+    const T *__restrict__ bpb = RECT ? (const T *)a.op1 : bpa;
+    const int vpa = a.nvir_pad, vpb = RECT ? a.nvir_pad_b : a.nvir_pad;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
-    const SplitQ sq(it, a.naux_pad, KP);
+    const SplitQ sq(it, a.naux_pad, KP, RECT);
     // Operand rows: 0 = (i, A), 1 = (j, B), 2 = (j, A), 3 = (i, B); for a
     // single-product item 2 and 3 are 0 and 1 again, one Q panel further on.
-    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
-    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
-    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
+    // Operands 0 and 2 come from bpa, 1 and 3 from bpb.
+    const int64_t base0 = (int64_t)it.i * vpa + A0;
+    const int64_t base1 = (int64_t)it.j * vpb + B0;
+    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * vpa + A0;
+    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * vpb + B0;
+    const int limA = vpa - A0, limB = vpb - B0;
     const int64_t ld = a.naux_pad;
     const int tid = threadIdx.x;
 
@@ -381,7 +419,7 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
             const int q = op < 2 ? q0 : q23;
             v4i v = {0, 0, 0, 0};
             if (c < NCH && row < lim && q < a.naux_pad)
-                v = *(const v4i *)(bp + (base + row) * ld + q + ch * EPC);
+                v = *(const v4i *)(((op & 1) ? bpb : bpa) + (base + row) * ld + q + ch * EPC);
             stg[t] = v;
         }
     };
@@ -402,8 +440,8 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
     bool rv[NB], cv[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
-        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
+        rv[k] = A0 + L.wr * TW + k * 16 < vpa;
+        cv[k] = B0 + L.wc * TW + k * 16 < vpb;
     }
 
     typename M::acc acc1[NB][NB], acc2[NB][NB];
@@ -429,7 +467,7 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
     };
     // FULL: every 16x16 block of the wave lies inside nvir_pad (the common
     // case), so the MFMA sequence needs no per-block test.
-    auto compute = [&](int buf, auto full) {
+    auto compute = [&](int buf, auto full, int q0) {
         frag_t f0[NB], f1[NB], f2[NB], f3[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
@@ -437,6 +475,14 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
             f2[k] = ldfrag(buf, 2, L.wr * TW + k * 16 + L.r);
             f1[k] = ldfrag(buf, 1, L.wc * TW + k * 16 + L.r);
             f3[k] = ldfrag(buf, 3, L.wc * TW + k * 16 + L.r);
+        }
+        if (q0 >= a.q_neg) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) f0[k] = -f0[k];
+        }
+        if (q0 + sq.off23 >= a.q_neg) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) f2[k] = -f2[k];
         }
 #pragma unroll
         for (int s = 0; s < KPL; ++s)
@@ -460,11 +506,19 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
         const int cur = p & 1;
         if (p + 1 < sq.iters) gload((p + 1) * sq.step);
         if (full)
-            compute(cur, std::true_type{});
+            compute(cur, std::true_type{}, p * sq.step);
         else
-            compute(cur, std::false_type{});
+            compute(cur, std::false_type{}, p * sq.step);
         if (p + 1 < sq.iters) lstore(cur ^ 1);
         __syncthreads();
+    }
+    if constexpr (RECT) {
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) acc1[x][y] += acc2[x][y];
+        rect_epilogue<TILE>(a, it, L, acc1, rv, cv, RowOf<T>{});
+        return;
     }
     if (sq.single) {
         static_assert(sizeof(lds) >= sizeof(T) * TW * (TILE + 1), "LDS holds a C1 half-tile");

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 2
+#define FERRIC_HIP_ABI_VERSION 3
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -232,6 +232,58 @@ fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom, const doub
                                size_t b_len, const double *eps_dev, size_t eps_len,
                                fh_precision precision, int use_graph, void *stream,
                                double *e_os, double *e_ss);
+
+/* ---- Unrestricted (open-shell) blocks and the signed factor ------------ */
+
+/* One spin channel of an unrestricted reference, as SpinChannel
+ * (crates/ferric-mp2/src/u_rimp2.rs:669-679): b is row-major
+ * (naux, nocc*nvir) -- a host const double* for fh_u_same_spin /
+ * fh_u_opposite_spin, a device pointer (FP64 or FP32) for the _dev entries;
+ * eps is the channel's full orbital-energy vector on the host, occupied i at
+ * eps[first_occ + i], virtual a at eps[nocc_total + a]. */
+typedef struct fh_spin_channel {
+    const void *b;
+    size_t nocc;
+    size_t nvir;
+    const double *eps;
+    size_t n_eps;
+    size_t first_occ;
+    size_t nocc_total;
+} fh_spin_channel;
+
+/* The drop-in for try_u_same_spin_on_device (rimp2_gpu.rs:884-889): the
+ * same-spin energy 1/4 sum_{i<=j} f_ij sum_ab (g_ab - g_ba)^2 / D of one
+ * channel (same_spin_block_energy, u_rimp2.rs:693-729). Any precision. */
+fh_status fh_u_same_spin(const fh_spin_channel *ch, size_t naux, fh_precision precision,
+                         double *energy);
+fh_status fh_u_same_spin_dev(const fh_spin_channel *ch, fh_dtype b_dtype, size_t naux,
+                             fh_precision precision, void *stream, double *energy);
+
+/* The drop-in for try_u_opposite_spin_on_device (rimp2_gpu.rs:893-902): the
+ * opposite-spin energy sum_{i,J} sum_{a,B} (ia|JB)^2 / D of an alpha and a beta
+ * channel fitted in one aux basis (opposite_spin_block_energy,
+ * u_rimp2.rs:731-760). precision: FH_FP64 or FH_FP32. */
+fh_status fh_u_opposite_spin(const fh_spin_channel *alpha, const fh_spin_channel *beta,
+                             size_t naux, fh_precision precision, double *energy);
+fh_status fh_u_opposite_spin_dev(const fh_spin_channel *alpha, const fh_spin_channel *beta,
+                                 fh_dtype b_dtype, size_t naux, fh_precision precision,
+                                 void *stream, double *energy);
+
+/* spin_components_from_signed_b_ov (rimp2.rs:1559-1607) on the device: the
+ * closed-shell spin components of K = pos^T pos - neg^T neg, pos
+ * (naux_pos, nocc*nvir) and neg (naux_neg, nocc*nvir) row-major (the terf
+ * kernel's Coulomb and terfc factors). The reference runs it on the CPU only;
+ * no kappa. naux_neg = 0 gives the plain energy of pos. precision: FH_FP64 or
+ * FH_FP32. The _dev entry takes device pointers of one dtype. */
+fh_status fh_rimp2_energy_signed(const double *pos, size_t naux_pos, const double *neg,
+                                 size_t naux_neg, size_t nocc, size_t nvir, const double *eps,
+                                 size_t n_eps, size_t first_occ, size_t nocc_total,
+                                 fh_precision precision, fh_spin_components *out);
+fh_status fh_rimp2_energy_signed_dev(const void *pos, size_t naux_pos, const void *neg,
+                                     size_t naux_neg, fh_dtype b_dtype, size_t nocc, size_t nvir,
+                                     const double *eps, size_t n_eps, size_t first_occ,
+                                     size_t nocc_total, fh_precision precision, void *stream,
+                                     fh_spin_components *out);
 
 #ifdef __cplusplus
 }

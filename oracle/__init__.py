@@ -66,6 +66,11 @@ def lib():
                                         ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         L.orc_energy_bound.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_u_same_spin.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, _sz, ctypes.c_int, _dp]
+        L.orc_u_opposite_spin.argtypes = [_dp, _dp, _sz, _sz, _sz, _sz, _sz, _dp, _sz, _sz, _sz,
+                                          _dp, _sz, _sz, _sz, ctypes.c_int, _dp]
+        L.orc_signed_spin_components.argtypes = [_dp, _sz, _dp, _sz, _sz, _sz, _dp, _sz, _sz,
+                                                 _sz, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -242,3 +247,52 @@ def energy_bound(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, kappa=None
     lib().orc_energy_bound(_p(b), naux, nocc, nvir, _p(e), first_occ, nocc_total, hk, k,
                            float(eps_g), float(eta), _p(out))
     return float(out[0]), float(out[1])
+
+
+def u_same_spin(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, nthreads: int = 0):
+    """Same-spin energy of one spin channel (u_rimp2.rs:693-729, 766-797)."""
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    out = np.zeros(1)
+    rc = lib().orc_u_same_spin(_p(b), b.shape[0], nocc, nvir, _p(e), e.size, first_occ,
+                               nocc_total, nthreads, _p(out))
+    if rc:
+        raise OracleError(f"oracle error {rc}")
+    return float(out[0])
+
+
+def u_opposite_spin(ch_a, ch_b, nthreads: int = 0):
+    """Opposite-spin energy of two channels (u_rimp2.rs:731-760, 978-995).
+
+    Each channel is (b_ov, eps, nocc, nvir, first_occ, nocc_total), both fitted
+    in one aux basis (same row count)."""
+    ba, ea, noa, nva, foa, nta = ch_a
+    bb, eb, nob, nvb, fob, ntb = ch_b
+    ba, ea, bb, eb = _f64(ba), _f64(ea), _f64(bb), _f64(eb)
+    if ba.shape[0] != bb.shape[0]:
+        raise OracleError("the two channels must share the aux basis")
+    out = np.zeros(1)
+    rc = lib().orc_u_opposite_spin(_p(ba), _p(bb), ba.shape[0], noa, nva, nob, nvb, _p(ea),
+                                   ea.size, foa, nta, _p(eb), eb.size, fob, ntb, nthreads,
+                                   _p(out))
+    if rc:
+        raise OracleError(f"oracle error {rc}")
+    return float(out[0])
+
+
+def signed_spin_components(pos, neg, eps, nocc, nvir, first_occ=0, nocc_total=None,
+                           nthreads: int = 0):
+    """(e_os, e_ss, e_total) of a signed factor K = pos^T pos - neg^T neg
+    (rimp2.rs:1559-1607)."""
+    bp, bn, e = _f64(pos), _f64(neg), _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    out = np.zeros(3)
+    rc = lib().orc_signed_spin_components(_p(bp), bp.shape[0], _p(bn), bn.shape[0], nocc, nvir,
+                                          _p(e), e.size, first_occ, nocc_total, nthreads,
+                                          _p(out))
+    if rc:
+        raise OracleError(f"oracle error {rc}")
+    return tuple(float(x) for x in out)

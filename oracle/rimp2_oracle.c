@@ -21,6 +21,12 @@
  *   - opposite_spin_density                crates/ferric-mp2/src/rimp2.rs:2026-2052
  *   - apply_spin_scaling / SpinScaling     crates/ferric-mp2/src/rimp2.rs:1917-2011
  *   - energy error bound (Higham model)    crates/ferric-mp2/tests/common/rimp2_error_bound.rs
+ *   - same_spin_block_energy / same_spin_energy_only
+ *                                          crates/ferric-mp2/src/u_rimp2.rs:693-729, 766-797
+ *   - opposite_spin_block_energy / opposite_spin_energy_only
+ *                                          crates/ferric-mp2/src/u_rimp2.rs:731-760, 978-995
+ *   - signed_pair_block / spin_components_from_signed_b_ov
+ *                                          crates/ferric-mp2/src/rimp2.rs:1559-1607
  *
  * The reference forms G_i with an OpenBLAS dgemm; here it is a plain blocked
  * loop in the same (P-outer) association class. The two agree to the
@@ -471,4 +477,181 @@ int orc_energy_bound(const double *b, size_t naux, size_t nocc, size_t nvir, con
     bound[0] = bos + gam * (2.0 * abs_os + bos);
     bound[1] = bss + gam * (2.0 * abs_ss + bss);
     return ORC_OK;
+}
+
+/* ---- Unrestricted blocks and the signed factor -------------------------- */
+
+/* same_spin_block_energy (u_rimp2.rs:693-729) on the j >= i tail block of one
+ * spin channel: 0.25 * sum_{j>=i} fac * sum_ab t*K, K = g_ab - g_ba,
+ * t = K/denom, loops j, a, b ascending. */
+static double u_same_spin_block(const double *g_i, size_t i, size_t nocc, size_t nvir,
+                                const double *eps, size_t first_occ, size_t nocc_total) {
+    const size_t ld = (nocc - i) * nvir;
+    const double eps_i = eps[first_occ + i];
+    double energy_i = 0.0;
+    for (size_t j = i; j < nocc; ++j) {
+        const double fac = (i == j) ? 1.0 : 2.0;
+        const size_t jcol = (j - i) * nvir;
+        const double eps_j = eps[first_occ + j];
+        double energy_ij = 0.0;
+        for (size_t a = 0; a < nvir; ++a) {
+            const double eps_a = eps[nocc_total + a];
+            for (size_t bb = 0; bb < nvir; ++bb) {
+                const double eps_b = eps[nocc_total + bb];
+                const double k = g_i[a * ld + jcol + bb] - g_i[bb * ld + jcol + a];
+                const double denom = eps_i + eps_j - eps_a - eps_b;
+                const double t_val = k / denom;
+                energy_ij += t_val * k;
+            }
+        }
+        energy_i += fac * energy_ij;
+    }
+    return 0.25 * energy_i;
+}
+
+/* same_spin_energy_only (u_rimp2.rs:766-797): per-i tail blocks, partials
+ * summed in ascending i. */
+int orc_u_same_spin(const double *b, size_t naux, size_t nocc, size_t nvir, const double *eps,
+                    size_t n_eps, size_t first_occ, size_t nocc_total, int nthreads,
+                    double *energy) {
+    if (check_layout(naux, nocc, nvir, n_eps, first_occ, nocc_total)) return ORC_LAYOUT;
+    *energy = 0.0;
+    if (nocc == 0 || nvir == 0) return ORC_OK;
+    double *part = (double *)calloc(nocc, sizeof(double));
+    if (!part) return ORC_ALLOC;
+    int failed = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (long ii = 0; ii < (long)nocc; ++ii) {
+        const size_t i = (size_t)ii;
+        double *g = (double *)malloc(sizeof(double) * nvir * (nocc - i) * nvir);
+        if (!g) {
+            failed = 1;
+            continue;
+        }
+        orc_tail_block(b, naux, nocc, nvir, i, g);
+        part[i] = u_same_spin_block(g, i, nocc, nvir, eps, first_occ, nocc_total);
+        free(g);
+    }
+    (void)nthreads;
+    if (!failed)
+        for (size_t i = 0; i < nocc; ++i) *energy += part[i];
+    free(part);
+    return failed ? ORC_ALLOC : ORC_OK;
+}
+
+/* opposite_spin_energy_only + opposite_spin_block_energy (u_rimp2.rs:978-995,
+ * 731-760): g_i = B_a[:, i*nvir_a..]^T B_b (nvir_a x nocc_b*nvir_b), then
+ * sum over a, J, B ascending of (eri/denom)*eri; per-i partials summed in
+ * ascending i. Both channels share one aux basis (naux rows). */
+int orc_u_opposite_spin(const double *b_a, const double *b_b, size_t naux, size_t nocc_a,
+                        size_t nvir_a, size_t nocc_b, size_t nvir_b, const double *eps_a,
+                        size_t n_eps_a, size_t first_occ_a, size_t nocc_total_a,
+                        const double *eps_b, size_t n_eps_b, size_t first_occ_b,
+                        size_t nocc_total_b, int nthreads, double *energy) {
+    if (check_layout(naux, nocc_a, nvir_a, n_eps_a, first_occ_a, nocc_total_a) ||
+        check_layout(naux, nocc_b, nvir_b, n_eps_b, first_occ_b, nocc_total_b))
+        return ORC_LAYOUT;
+    *energy = 0.0;
+    if (nocc_a == 0 || nvir_a == 0 || nocc_b == 0 || nvir_b == 0) return ORC_OK;
+    const size_t nov_a = nocc_a * nvir_a, nov_b = nocc_b * nvir_b;
+    double *part = (double *)calloc(nocc_a, sizeof(double));
+    if (!part) return ORC_ALLOC;
+    int failed = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (long ii = 0; ii < (long)nocc_a; ++ii) {
+        const size_t i = (size_t)ii;
+        double *g = (double *)calloc(nvir_a * nov_b, sizeof(double));
+        if (!g) {
+            failed = 1;
+            continue;
+        }
+        for (size_t p = 0; p < naux; ++p) {
+            const double *ra = b_a + p * nov_a + i * nvir_a;
+            const double *rb = b_b + p * nov_b;
+            for (size_t a = 0; a < nvir_a; ++a) {
+                const double s = ra[a];
+                double *gr = g + a * nov_b;
+                for (size_t c = 0; c < nov_b; ++c) gr[c] += s * rb[c];
+            }
+        }
+        const double eps_i = eps_a[first_occ_a + i];
+        double energy_i = 0.0;
+        for (size_t a = 0; a < nvir_a; ++a) {
+            const double eps_av = eps_a[nocc_total_a + a];
+            for (size_t jj = 0; jj < nocc_b; ++jj) {
+                const double eps_j = eps_b[first_occ_b + jj];
+                for (size_t bb = 0; bb < nvir_b; ++bb) {
+                    const double eps_bv = eps_b[nocc_total_b + bb];
+                    const double eri = g[a * nov_b + jj * nvir_b + bb];
+                    const double denom = eps_i + eps_j - eps_av - eps_bv;
+                    const double t_val = eri / denom;
+                    energy_i += t_val * eri;
+                }
+            }
+        }
+        part[i] = energy_i;
+        free(g);
+    }
+    (void)nthreads;
+    if (!failed)
+        for (size_t i = 0; i < nocc_a; ++i) *energy += part[i];
+    free(part);
+    return failed ? ORC_ALLOC : ORC_OK;
+}
+
+/* signed_pair_block + spin_components_from_signed_b_ov (rimp2.rs:1559-1607):
+ * G_i = pos_i^T pos_tail - neg_i^T neg_tail over the j >= i tail, then the
+ * closed-shell pair loop and ascending fold of spin_components_from_blocks
+ * (no kappa). pos is (naux_p, nocc*nvir), neg is (naux_n, nocc*nvir). */
+int orc_signed_spin_components(const double *pos, size_t naux_p, const double *neg,
+                               size_t naux_n, size_t nocc, size_t nvir, const double *eps,
+                               size_t n_eps, size_t first_occ, size_t nocc_total, int nthreads,
+                               double *out) {
+    if (check_layout(naux_p, nocc, nvir, n_eps, first_occ, nocc_total)) return ORC_LAYOUT;
+    out[0] = out[1] = out[2] = 0.0;
+    if (nocc == 0 || nvir == 0) return ORC_OK;
+    double *part = (double *)calloc(2 * nocc, sizeof(double));
+    if (!part) return ORC_ALLOC;
+    int failed = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (long ii = 0; ii < (long)nocc; ++ii) {
+        const size_t i = (size_t)ii;
+        const size_t n = nvir * (nocc - i) * nvir;
+        double *g = (double *)malloc(sizeof(double) * n);
+        double *gn = (double *)malloc(sizeof(double) * n);
+        if (!g || !gn) {
+            free(g);
+            free(gn);
+            failed = 1;
+            continue;
+        }
+        orc_tail_block(pos, naux_p, nocc, nvir, i, g);
+        orc_tail_block(neg, naux_n, nocc, nvir, i, gn);
+        for (size_t k = 0; k < n; ++k) g[k] -= gn[k];
+        orc_i_partial(g, i, nocc, nvir, eps, first_occ, nocc_total, 0, 0.0, &part[2 * i],
+                      &part[2 * i + 1]);
+        free(g);
+        free(gn);
+    }
+    (void)nthreads;
+    double os = 0.0, ss = 0.0;
+    if (!failed)
+        for (size_t i = 0; i < nocc; ++i) {
+            os += part[2 * i];
+            ss += part[2 * i + 1];
+        }
+    free(part);
+    out[0] = os;
+    out[1] = ss;
+    out[2] = os + ss;
+    return failed ? ORC_ALLOC : ORC_OK;
 }

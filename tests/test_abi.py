@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.fh_abi_version() == 2
+    assert L.fh_abi_version() == 3
 
 
 def test_enum_values_match_header():
@@ -107,3 +107,39 @@ def test_try_on_device_returns_none_on_failure(monkeypatch):
     assert rimp2.try_spin_components_on_device(np.zeros((4, 6)), np.zeros(3), 2, 3, 0, 2) is None
     monkeypatch.setenv("FERRIC_GPU", "off")
     assert rimp2.try_spin_components_on_device(np.zeros((4, 0)), np.zeros(3), 0, 3, 0, 0) is None
+
+
+def test_spin_channel_layout():
+    # fh_spin_channel: pointer, 2 x size_t, pointer, 3 x size_t
+    assert ctypes.sizeof(_lib.SpinChannelC) == 7 * 8
+
+
+def test_unrestricted_argument_checks_before_device():
+    from ferric_amd import unrestricted as u
+
+    eps = np.zeros(10)
+    ch = _lib.SpinChannelC(None, 2, 3, eps.ctypes.data_as(_lib._c_dp), eps.size, 0, 2)
+    out = ctypes.c_double()
+    L = _lib.lib()
+    # unsupported precision for the opposite-spin rectangle
+    st = L.fh_u_opposite_spin(ctypes.byref(ch), ctypes.byref(ch), 4, 2, ctypes.byref(out))
+    assert st == _lib.Status.UNSUPPORTED
+    # eps too short (needs nocc_total + nvir = 5 here it is 10, so shrink)
+    short = _lib.SpinChannelC(None, 2, 3, eps.ctypes.data_as(_lib._c_dp), 4, 0, 2)
+    st = L.fh_u_same_spin(ctypes.byref(short), 4, 0, ctypes.byref(out))
+    assert st == _lib.Status.LAYOUT
+    # empty channel: zero energy without a device
+    empty = _lib.SpinChannelC(None, 0, 3, eps.ctypes.data_as(_lib._c_dp), eps.size, 0, 0)
+    st = L.fh_u_same_spin(ctypes.byref(empty), 4, 0, ctypes.byref(out))
+    assert st == _lib.Status.OK and out.value == 0.0
+    # the Python layer refuses channels fitted in different aux bases
+    with pytest.raises(_lib.LayoutError):
+        u.u_opposite_spin(u.SpinChannel(np.zeros((4, 6)), eps, 2, 3),
+                          u.SpinChannel(np.zeros((5, 6)), eps, 2, 3))
+    # signed: FP8 is refused before any device work
+    sc = _lib.SpinComponentsC()
+    b = np.zeros((4, 6))
+    st = L.fh_rimp2_energy_signed(b.ctypes.data_as(_lib._c_dp), 4, b.ctypes.data_as(_lib._c_dp),
+                                  4, 2, 3, eps.ctypes.data_as(_lib._c_dp), eps.size, 0, 2, 3,
+                                  ctypes.byref(sc))
+    assert st == _lib.Status.UNSUPPORTED
