@@ -339,6 +339,45 @@ def main():
             f"{adj_small['BF16X3']['max_rel_dGamma']:.1e}")
         del db, wb, outs, g64, g16
 
+        # Open shell at decane scale (alpha 21 occ / 229 vir, beta 20 / 230, naux
+        # 960, same distributions): same-spin blocks through the pair kernel and
+        # the opposite-spin rectangle, FP64; and a signed factor (960 + 480 rows).
+        from ferric_amd import unrestricted as un
+
+        rng = np.random.default_rng(1006)
+
+        def chan(no, nv):
+            bb = torch.from_numpy(rng.standard_normal((960, no * nv)) * math.sqrt(0.09 / 960)).to(dev)
+            occ = np.sort(np.concatenate([rng.uniform(-11.3, -11.1, 5),
+                                          rng.uniform(-1.2, -0.35, no - 5)]))
+            vir = np.sort(np.exp(rng.uniform(math.log(0.12), math.log(5.0), nv)))
+            return un.SpinChannel(bb, np.concatenate([occ, vir]), no, nv)
+
+        ca, cb = chan(21, 229), chan(20, 230)
+        uhf = {}
+        for name, fn, fl in (
+                ("same_spin_alpha", lambda: un.u_same_spin(ca), 21 * 22 * 229 ** 2 * 960),
+                ("same_spin_beta", lambda: un.u_same_spin(cb), 20 * 21 * 230 ** 2 * 960),
+                ("opposite_spin", lambda: un.u_opposite_spin(ca, cb),
+                 2 * 21 * 20 * 229 * 230 * 960)):
+            val = fn()
+            med, mn = timed_ms(fn, 10)
+            uhf[name] = {"E": val, "ms_median": med, "ms_min": mn,
+                         "tflops": fl / (med * 1e-3) / 1e12}
+        small["uhf 21+20 occ, 229/230 vir, 960 aux (FP64)"] = uhf
+        log("uhf: " + ", ".join(f"{k} {v['ms_median']:.2f} ms" for k, v in uhf.items()))
+        pos = torch.from_numpy(dc.b_ov).to(dev)
+        neg = torch.from_numpy(rng.standard_normal((480, dc.nocc * dc.nvir))
+                               * math.sqrt(0.02 / 480)).to(dev)
+        sg = un.spin_components_from_signed_b_ov(pos, neg, dc.eps, dc.nocc, dc.nvir)
+        med, mn = timed_ms(lambda: un.spin_components_from_signed_b_ov(pos, neg, dc.eps, dc.nocc,
+                                                                       dc.nvir), 10)
+        small[dc.name + " signed (960 + 480 aux rows, FP64)"] = {
+            "E_os": sg.e_os, "E_ss": sg.e_ss, "ms_median": med, "ms_min": mn,
+            "tflops": dc.flops * 1440 / 960 / (med * 1e-3) / 1e12}
+        log(f"signed decane: {med:.2f} ms")
+        del pos, neg, ca, cb
+
     ok = [v for v in variants if v != Precision.FP64 and abs(res[v.name]["dE_vs_fp64"]) <= 1e-6]
     head = min(ok, key=lambda v: res[v.name]["ms_per_step"]) if ok else Precision.FP64
     h = res[head.name]
