@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -373,52 +374,115 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g) {
 // ---- Row reduction of one batch: energy and dE/deps -------------------------
 struct AdjRed {
     int i, nocc, nvir, tile, ntile, ntp, vec_stride;
+    int nchunks, jc;              // pair chunks of the virtual sums (jc pairs each)
     const double *item_vec;       // (nocc - i) * ntp items
     const double *item_partials;  // 4 per item
+    double *vpart;                // (nchunks, nvir, 2) virtual partial sums of this row
+    double *ppart;                // (npair, 6) per-pair sums of this row
     double *rows;                 // 4 per occupied row
     double *dvir_s, *dvir_c;      // dE/deps_vir, compensated, accumulated over batches
     double *docc_s, *docc_c;      // dE/deps_occ
 };
 
+// Reduction of one row batch, in two launches with a fixed order:
+//   adjoint_reduce_kernel: blocks [0, nvb*nchunks) sum, for 256 virtual
+//     indices each, the t*T~ row and column sums of one chunk of jc pairs
+//     (f = 1 for j = i, else 2) into vpart; blocks from nvb*nchunks on take one
+//     pair each and sum its items' energy partials and t*T~ totals into ppart
+//     (strided per thread, then a fixed tree);
+//   adjoint_fold_kernel: adds the chunks into dE/deps_vir in chunk order, and
+//     one thread folds the pairs in j order into the row's energy and the
+//     occupied derivatives (dE/deps_i and dE/deps_j: -2 sum_ab t T~ per pair).
 __global__ __launch_bounds__(256) void adjoint_reduce_kernel(AdjRed r) {
     const int npair = r.nocc - r.i;
-    if (blockIdx.x < gridDim.x - 1) {
-        // dE/deps_c = sum over unique pairs f_ij (row sums + column sums).
-        const int c = blockIdx.x * 256 + threadIdx.x;
+    const int nvb = (r.nvir + 255) / 256;
+    const int b = blockIdx.x;
+    if (b < nvb * r.nchunks) {
+        const int ch = b / nvb;
+        const int c = (b % nvb) * 256 + threadIdx.x;
         if (c >= r.nvir) return;
         const int tc = c / r.tile, off = c % r.tile;
-        KSum s;
-        s.s = r.dvir_s[c];
-        s.c = r.dvir_c[c];
-        // Tile pairs touching tile tc: (tc, u >= tc) by rows, (u <= tc, tc) by columns.
         auto tp_of = [&](int ta, int tb) { return ta * r.ntile - ta * (ta - 1) / 2 + (tb - ta); };
-        for (int jj = 0; jj < npair; ++jj) {
+        KSum s;
+        const int j1 = min(npair, (ch + 1) * r.jc);
+        for (int jj = ch * r.jc; jj < j1; ++jj) {
             const double f = jj == 0 ? 1.0 : 2.0;
             const double *base = r.item_vec + (int64_t)jj * r.ntp * r.vec_stride;
+            // tile pairs touching tile tc: (tc, u >= tc) by rows, (u <= tc, tc) by columns
             for (int u = 0; u < r.ntile; ++u) {
                 if (u >= tc) s.add(f * base[(int64_t)tp_of(tc, u) * r.vec_stride + off]);
                 if (u <= tc) s.add(f * base[(int64_t)tp_of(u, tc) * r.vec_stride + r.tile + off]);
             }
         }
-        r.dvir_s[c] = s.s;
-        r.dvir_c[c] = s.c;
+        double *o = r.vpart + 2 * ((int64_t)ch * r.nvir + c);
+        o[0] = s.s;
+        o[1] = s.c;
         return;
     }
-    // Last block: this row's energy and the occupied derivatives.
-    if (threadIdx.x != 0) return;
+    const int jj = b - nvb * r.nchunks;
+    if (jj >= npair) return;
+    KSum eos, ess, sp;
+    for (int t = threadIdx.x; t < r.ntp; t += 256) {
+        const int64_t item = (int64_t)jj * r.ntp + t;
+        const double *ip = r.item_partials + 4 * item;
+        eos.merge(ip[0], ip[1]);
+        ess.merge(ip[2], ip[3]);
+        sp.add(r.item_vec[item * r.vec_stride + 2 * r.tile]);
+    }
+    wave_merge(eos);
+    wave_merge(ess);
+    wave_merge(sp);
+    __shared__ double red[4][6];
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane == 0) {
+        red[wid][0] = eos.s;
+        red[wid][1] = eos.c;
+        red[wid][2] = ess.s;
+        red[wid][3] = ess.c;
+        red[wid][4] = sp.s;
+        red[wid][5] = sp.c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KSum a, e, p;
+        for (int w = 0; w < 4; ++w) {
+            a.merge(red[w][0], red[w][1]);
+            e.merge(red[w][2], red[w][3]);
+            p.merge(red[w][4], red[w][5]);
+        }
+        double *o = r.ppart + 6 * (int64_t)jj;
+        o[0] = a.s;
+        o[1] = a.c;
+        o[2] = e.s;
+        o[3] = e.c;
+        o[4] = p.s;
+        o[5] = p.c;
+    }
+}
+
+__global__ __launch_bounds__(256) void adjoint_fold_kernel(AdjRed r) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c < r.nvir) {
+        KSum s;
+        s.s = r.dvir_s[c];
+        s.c = r.dvir_c[c];
+        for (int ch = 0; ch < r.nchunks; ++ch) {
+            const double *o = r.vpart + 2 * ((int64_t)ch * r.nvir + c);
+            s.merge(o[0], o[1]);
+        }
+        r.dvir_s[c] = s.s;
+        r.dvir_c[c] = s.c;
+    }
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int npair = r.nocc - r.i;
     KSum eos, ess, di;
     di.s = r.docc_s[r.i];
     di.c = r.docc_c[r.i];
     for (int jj = 0; jj < npair; ++jj) {
-        double spair = 0.0;
-        for (int t = 0; t < r.ntp; ++t) {
-            const int64_t item = (int64_t)jj * r.ntp + t;
-            const double *ip = r.item_partials + 4 * item;
-            eos.merge(ip[0], ip[1]);
-            ess.merge(ip[2], ip[3]);
-            spair += r.item_vec[item * r.vec_stride + 2 * r.tile];
-        }
-        // dE/deps_i and dE/deps_j: -2 sum_ab t T~ of the pair (once for j = i).
+        const double *o = r.ppart + 6 * (int64_t)jj;
+        eos.merge(o[0], o[1]);
+        ess.merge(o[2], o[3]);
+        const double spair = o[4] + o[5];
         di.add(-2.0 * spair);
         if (jj > 0) {
             const int j = r.i + jj;
@@ -495,6 +559,20 @@ hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st) {
 
 // ---- Host orchestration ------------------------------------------------------
 
+// Output tile of the phase-B/C GEMMs: 64 (several workgroups per CU) for both
+// precisions -- at 100/900/3600 the energy + adjoint takes 1932 ms (FP64) and
+// 692 ms (BF16x3) with 64-wide tiles against 2194 and 837 ms with 128-wide
+// ones, whose FP64 kernel holds one wave per SIMD. FERRIC_HIP_GEMM_TILE (64 or
+// 128) overrides.
+static int gemm_tile(Prec prec) {
+    (void)prec;
+    if (const char *e = std::getenv("FERRIC_HIP_GEMM_TILE")) {
+        const int t = std::atoi(e);
+        if (t == 64 || t == 128) return t;
+    }
+    return 64;
+}
+
 static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux,
                               size_t nocc, size_t nvir, const double *eps, size_t n_eps,
                               size_t first_occ, size_t nocc_total, double c_os, double c_ss,
@@ -503,12 +581,15 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     const bool in_f32 = b_dtype == FH_DT_F32;
     const bool f64 = prec == Prec::F64;
     EnergyPlan pl = plan_energy(nocc, nvir, naux, prec);
+    // The FP64 adjoint epilogue does not fit a 128-wide tile in the register
+    // file (it would spill); 64-wide tiles run two workgroups per CU.
+    if (f64 && pl.tile > 64) pl.tile = 64;
     // K dimensions of the phase-B/C GEMMs run over padded virtuals in panels of
     // 32 (BF16) or 8 (FP64): pad the virtual block to 32.
     pl.nvir_pad = (int)((nvir + 31) / 32 * 32);
     pl.ntile = (pl.nvir_pad + pl.tile - 1) / pl.tile;
     pl.ntp = pl.ntile * (pl.ntile + 1) / 2;
-    const int gtile = 128;
+    const int gtile = gemm_tile(prec);
     const int m_pad = (int)((naux + gtile - 1) / gtile * gtile);
     const size_t rows = nocc * (size_t)pl.nvir_pad;
     const size_t eb = packed_elem_bytes(prec);
@@ -527,6 +608,10 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     int splits = (int)std::min<int64_t>((512 + nbB - 1) / nbB, std::max<int64_t>(1, k0 / 256));
     splits = std::max(1, splits);
     const size_t items_max = nocc * (size_t)pl.ntp;
+    // Row reductions: virtual sums in chunks of pairs (enough blocks to spread
+    // over the CUs), one block per pair for the energy and occupied sums.
+    const int nvb = (int)((nvir + 255) / 256);
+    const int nchunks_max = (int)std::min<size_t>(nocc, (size_t)std::max(1, (512 + nvb - 1) / nvb));
     std::vector<std::pair<int, size_t>> req = {
         {kSlotOp0, op_bytes},
         {kSlotStage2, b2_bytes},
@@ -540,6 +625,7 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
         {kSlotResult, sizeof(double) * (2 + nocc + nvir)},
         {kSlotScalar, 256},
         {kSlotSplit, sizeof(double) * (size_t)splits * m_pad * pl.nvir_pad},
+        {kSlotRed, sizeof(double) * (2 * (size_t)nchunks_max * nvir + 6 * nocc)},
     };
     if (!f64) {
         req.push_back({kSlotOp1, op_bytes});
@@ -670,7 +756,8 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     rd.docc_c = acc + nocc;
     rd.dvir_s = acc + 2 * nocc;
     rd.dvir_c = acc + 2 * nocc + nvir;
-    const unsigned red_blocks = (unsigned)((nvir + 255) / 256 + 1);
+    rd.vpart = (double *)s.slots[kSlotRed].ptr;
+    rd.ppart = rd.vpart + 2 * (size_t)nchunks_max * nvir;
 
     int64_t total_items = 0;
     for (size_t i = 0; i < nocc; ++i) {
@@ -680,7 +767,15 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
         he = launch_adjoint_pairs(ea, stream);
         if (he != hipSuccess) return fail(FH_KERNEL, "adjoint pair kernel: %s", hipGetErrorString(he));
         rd.i = (int)i;
-        hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(red_blocks), dim3(256), 0, stream, rd);
+        {
+            const int npair = (int)(nocc - i);
+            rd.nchunks = std::min(npair, nchunks_max);
+            rd.jc = (npair + rd.nchunks - 1) / rd.nchunks;
+            rd.nchunks = (npair + rd.jc - 1) / rd.jc;
+            hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)(nvb * rd.nchunks + npair)),
+                               dim3(256), 0, stream, rd);
+            hipLaunchKernelGGL(adjoint_fold_kernel, dim3((unsigned)nvb), dim3(256), 0, stream, rd);
+        }
         ga.i = (int)i;
         {
             // Same split count for every row (fixed summation structure); the
@@ -702,7 +797,7 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     he = hipGetLastError();
     if (he != hipSuccess) return fail(FH_KERNEL, "adjoint reduction: %s", hipGetErrorString(he));
     mark(s, 3, stream);
-    s.pending.kernel_launches += 4 * nocc + 6;
+    s.pending.kernel_launches += 5 * nocc + 6;
     std::vector<double> res(2 + nocc + nvir);
     st = hip_check(hipMemcpyAsync(res.data(), ea.result, sizeof(double) * res.size(),
                                   hipMemcpyDeviceToHost, stream),

@@ -55,10 +55,10 @@ __device__ __forceinline__ pair::Item decode_item(const EnergyArgs &a) {
     return it;
 }
 
-template <typename T, int TILE, int KP, bool ADJ>
+template <typename T, int TILE, int KP, bool ADJ, bool SIGNED = false>
 __global__ __launch_bounds__(256) void pair_k4_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::K4Lds<T, TILE, KP> lds;
-    pair::pair_k4_body<T, TILE, KP, ADJ>(a, decode_item(a), lds);
+    pair::pair_k4_body<T, TILE, KP, ADJ, false, SIGNED>(a, decode_item(a), lds);
 }
 
 // Opposite-spin rectangle: item (i, J, tA, tB), J and tB on the beta side;
@@ -141,6 +141,18 @@ hipError_t launch_items(K kern, const EnergyArgs &a, hipStream_t st) {
 hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
     if (a.nitems <= 0 || a.nitems > 0x7fffffffLL) return hipErrorInvalidValue;
     hipError_t e = hipErrorInvalidValue;
+    if (a.q_neg < a.naux_pad) {  // signed factor: FP64 / FP32 only
+        if (a.prec == Prec::F64) {
+            if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 16, false, true>, a, st);
+            else if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 16, false, true>, a, st);
+            else if (a.tile == 32) e = launch_items(pair_k4_kernel<double, 32, 16, false, true>, a, st);
+        } else if (a.prec == Prec::F32) {
+            if (a.tile == 128) e = launch_items(pair_k4_kernel<float, 128, 16, false, true>, a, st);
+            else if (a.tile == 64) e = launch_items(pair_k4_kernel<float, 64, 16, false, true>, a, st);
+            else if (a.tile == 32) e = launch_items(pair_k4_kernel<float, 32, 16, false, true>, a, st);
+        }
+        return e;
+    }
     switch (a.prec) {
     case Prec::F64:
         if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 16, false>, a, st);
@@ -169,8 +181,8 @@ hipError_t launch_adjoint_pairs(const EnergyArgs &a, hipStream_t st) {
     hipError_t e = hipErrorInvalidValue;
     switch (a.prec) {
     case Prec::F64:
-        if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 8, true>, a, st);
-        else if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 8, true>, a, st);
+        // 32 or 64 wide (the host caps the FP64 adjoint tile at 64)
+        if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 16, true>, a, st);
         else if (a.tile == 32) e = launch_items(pair_k4_kernel<double, 32, 8, true>, a, st);
         break;
     case Prec::BF16X3:

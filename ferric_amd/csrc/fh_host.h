@@ -32,6 +32,7 @@ enum SlotId {
     kSlotU,          // adjoint: b-major T~ blocks of one row
     kSlotAcc,        // adjoint: dE/deps accumulators
     kSlotSplit,      // adjoint: split-K partial tiles of phase B
+    kSlotRed,        // adjoint: per-row reduction partials
     kNumSlots
 };
 

@@ -370,10 +370,11 @@ using K4Lds = T[2][4][TILE][KP + 16 / sizeof(T)];
 
 // RECT: the opposite-spin rectangle (op0 alpha rows, op1 beta rows, it.j the
 // beta occupied index); every item is single-product and ends in
-// rect_epilogue. Panels whose first Q column lies at or past a.q_neg belong to
-// the negative factor of a signed kernel: their A-side fragments are negated,
-// so the MFMA chain accumulates B+^T B+ - B-^T B-.
-template <typename T, int TILE, int KP, bool ADJ, bool RECT = false>
+// rect_epilogue. SIGNED: panels whose first Q column lies at or past a.q_neg
+// belong to the negative factor of a signed kernel; their A-side fragments are
+// negated, so the MFMA chain accumulates B+^T B+ - B-^T B- (a compile-time
+// switch, so the other kernels carry no extra registers for it).
+template <typename T, int TILE, int KP, bool ADJ, bool RECT = false, bool SIGNED = false>
 __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it,
                                              K4Lds<T, TILE, KP> &lds) {
     using M = Mfma4<T>;
@@ -476,13 +477,15 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
             f1[k] = ldfrag(buf, 1, L.wc * TW + k * 16 + L.r);
             f3[k] = ldfrag(buf, 3, L.wc * TW + k * 16 + L.r);
         }
-        if (q0 >= a.q_neg) {
+        if constexpr (SIGNED) {
+            if (q0 >= a.q_neg) {
 #pragma unroll
-            for (int k = 0; k < NB; ++k) f0[k] = -f0[k];
-        }
-        if (q0 + sq.off23 >= a.q_neg) {
+                for (int k = 0; k < NB; ++k) f0[k] = -f0[k];
+            }
+            if (q0 + sq.off23 >= a.q_neg) {
 #pragma unroll
-            for (int k = 0; k < NB; ++k) f2[k] = -f2[k];
+                for (int k = 0; k < NB; ++k) f2[k] = -f2[k];
+            }
         }
 #pragma unroll
         for (int s = 0; s < KPL; ++s)
