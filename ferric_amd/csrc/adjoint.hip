@@ -402,7 +402,7 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(AdjRed r) {
         const int c = (b % nvb) * 256 + threadIdx.x;
         if (c >= r.nvir) return;
         const int tc = c / r.tile, off = c % r.tile;
-        auto tp_of = [&](int ta, int tb) { return ta * r.ntile - ta * (ta - 1) / 2 + (tb - ta); };
+        auto tp_of = [&](int ta, int tb) { return encode_tile_pair(ta, tb, r.ntile); };
         KSum s;
         const int j1 = min(npair, (ch + 1) * r.jc);
         for (int jj = ch * r.jc; jj < j1; ++jj) {

@@ -73,15 +73,30 @@ __device__ __forceinline__ void decode_pair(int64_t p, int nocc, int &i, int &j)
     j = ii + (int)rem;
 }
 
-// Tile pair t over nt tiles with tA <= tB (row-major upper triangle).
+// Tile pair t over nt tiles with tA <= tB: the nt(nt-1)/2 off-diagonal pairs
+// first (row-major strict upper triangle), then the nt diagonal ones. The
+// diagonal items do half the work of the others (single-product items), so
+// putting them last lets the workgroups that run one occupied pair's items
+// side by side stay in step and share its operand panels in L2.
 __device__ __forceinline__ void decode_tile_pair(int t, int nt, int &ta, int &tb) {
+    const int noff = nt * (nt - 1) / 2;
+    if (t >= noff) {
+        ta = tb = t - noff;
+        return;
+    }
     int a = 0, rem = t;
-    while (rem >= nt - a) {
-        rem -= nt - a;
+    while (rem >= nt - 1 - a) {
+        rem -= nt - 1 - a;
         ++a;
     }
     ta = a;
-    tb = a + rem;
+    tb = a + 1 + rem;
+}
+
+// Inverse of decode_tile_pair.
+__device__ __forceinline__ int encode_tile_pair(int ta, int tb, int nt) {
+    if (ta == tb) return nt * (nt - 1) / 2 + ta;
+    return ta * (nt - 1) - ta * (ta - 1) / 2 + (tb - ta - 1);
 }
 
 // Wave-level butterfly reduction of a compensated pair (deterministic order).
