@@ -29,7 +29,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"FP64": 78.6, "FP32": 157.3, "BF16X3": 2500.0 / 3.0, "FP8_E4M3": 5000.0}
+# Useful-FLOP peaks: BF16x3 issues 3 MFMA products per useful one, INT8X4 13
+# INT8 digit products (dense INT8 5.0 POP/s).
+PEAK_TFLOPS = {"FP64": 78.6, "FP32": 157.3, "BF16X3": 2500.0 / 3.0, "FP8_E4M3": 5000.0,
+               "INT8X4": 5000.0 / 13.0}
+ISSUED_PER_USEFUL = {"BF16X3": 3, "INT8X4": 13}
 HBM_GBS = 8000.0
 
 
@@ -51,7 +55,8 @@ PROBE_LIB = os.path.join(ROOT, "tests", "hip", "build", "libmfma_probe.so")
 # probe kind, iterations per wave for each headline precision (about 30 ms per
 # launch); FP64 uses 8 accumulators per wave (three waves per SIMD), the
 # fastest f64 MFMA issue pattern
-RATE_PROBE = {"FP64": (5, 16000), "BF16X3": (2, 30000), "FP8_E4M3": (3, 15000)}
+RATE_PROBE = {"FP64": (5, 16000), "BF16X3": (2, 30000), "FP8_E4M3": (3, 15000),
+              "INT8X4": (14, 30000)}
 # every probe kind of tests/hip/mfma_probe.hip: (kind, label, iterations)
 ALL_PROBES = [
     (0, "f64 MFMA 16x16x4, 16 acc (1 wave/SIMD)", 8000),
@@ -65,6 +70,7 @@ ALL_PROBES = [
     (13, "f64 vector FMA from LDS, 8x8 interleaved, 3 WG/CU", 1500),
     (2, "bf16 MFMA 16x16x32", 30000),
     (3, "fp8 scaled MFMA 16x16x128", 15000),
+    (14, "i8 MFMA 16x16x64", 30000),
 ]
 # rocprofv3 summaries holding the measured HBM bytes per launch of a headline kernel
 TRAFFIC_PROFILE = {
@@ -107,8 +113,7 @@ def mfma_sustained(prec_name):
     rate = probe_rate(*RATE_PROBE[prec_name])
     if rate is None:
         return None
-    if prec_name == "BF16X3":
-        rate /= 3.0   # three MFMAs per useful product
+    rate /= ISSUED_PER_USEFUL.get(prec_name, 1)   # MFMA products per useful one
     return rate
 
 
@@ -133,7 +138,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "decane", "water"])
-    ap.add_argument("--variants", default="FP64,FP32,BF16X3,FP8_E4M3")
+    ap.add_argument("--variants", default="FP64,FP32,BF16X3,FP8_E4M3,INT8X4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -221,8 +226,8 @@ def main():
         r.update(ms_per_step=ms, contract_ms=c_ms, pack_ms=p_ms, reduce_ms=r_ms, tile=tile,
                  tflops=F / (ms * 1e-3) / 1e12, kernel_tflops=F / (c_ms * 1e-3) / 1e12,
                  dE_vs_fp64=r["E"] - e_ref)
-        if prec == Precision.BF16X3:
-            r["kernel_tflops_mfma_issued"] = 3 * F / (c_ms * 1e-3) / 1e12
+        if prec.name in ISSUED_PER_USEFUL:
+            r["kernel_tflops_mfma_issued"] = ISSUED_PER_USEFUL[prec.name] * F / (c_ms * 1e-3) / 1e12
         log(f"{prec.name}: {ms:.2f} ms/step (kernel {c_ms:.2f} ms, pack {p_ms:.2f}, reduce "
             f"{r_ms:.3f}) {r['tflops']:.1f} TFLOP/s  dE {r['dE_vs_fp64']:+.3e}")
 

@@ -37,6 +37,7 @@ class Precision(enum.IntEnum):
     FP32 = 1
     BF16X3 = 2
     FP8_E4M3 = 3
+    INT8X4 = 4
 
 
 class DType(enum.IntEnum):
@@ -54,7 +55,7 @@ class StatsC(ctypes.Structure):
         "calls", "offloaded", "fp64_calls", "fp32_calls", "bf16x3_calls", "fp8_calls",
         "adjoint_calls", "batched_calls", "h2d_bytes", "d2h_bytes", "kernel_launches",
         "fail_layout", "fail_pool_full", "fail_hip", "fail_range", "fail_unsupported",
-        "pool_peak_bytes")]
+        "pool_peak_bytes", "int8x4_calls")]
 
 
 class TimingC(ctypes.Structure):
@@ -139,6 +140,8 @@ SIGNATURES = {
                                              ctypes.POINTER(SpinComponentsC), _c_vp, _c_dp]),
     "fh_rimp2_batched_dev": (_c_int, [ctypes.POINTER(DomainC), _c_sz, _c_vp, _c_sz, _c_vp,
                                       _c_sz, _c_int, _c_int, _c_vp, _c_dp, _c_dp]),
+    "fh_dress_b_ov_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz,
+                                   _c_vp, _c_vp]),
     "fh_u_same_spin": (_c_int, [ctypes.POINTER(SpinChannelC), _c_sz, _c_int, _c_dp]),
     "fh_u_same_spin_dev": (_c_int, [ctypes.POINTER(SpinChannelC), _c_int, _c_sz, _c_int, _c_vp,
                                     _c_dp]),
@@ -155,7 +158,7 @@ SIGNATURES = {
 }
 
 _lib = None
-ABI_VERSION = 3   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 4   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

@@ -172,6 +172,9 @@ hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
     case Prec::FP8:
         e = launch_energy_fp8(a, st);
         break;
+    case Prec::I8X4:
+        e = launch_energy_i8x4(a, st);
+        break;
     }
     return e;
 }

@@ -232,6 +232,7 @@ size_t packed_elem_bytes(Prec p) {
     case Prec::F32: return 4;
     case Prec::BF16X3: return 2;  // per array (hi and lo)
     case Prec::FP8: return 1;
+    case Prec::I8X4: return 4;  // four digit bytes
     }
     return 8;
 }
@@ -298,10 +299,20 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
         req.push_back({kSlotDiag, sizeof(double) * rows});
     }
     if (prec == Prec::FP8) req.push_back({kSlotScale, nocc * nqb});
+    if (prec == Prec::I8X4) {
+        // The int32 level sums bound naux (their own range limit, like the
+        // FP32 one below): beyond it the caller retries in FP64.
+        if (pl.naux_pad > kI8MaxNauxPad)
+            return fail(FH_RANGE, "naux_pad %d beyond the INT8 slices' int32 range (%d)",
+                        pl.naux_pad, kI8MaxNauxPad);
+        req.push_back({kSlotScale, sizeof(int) * rows});
+    }
     fh_status st = reserve(s, req);
     if (st != FH_OK) return st;
 
-    if (prec != Prec::F64) {
+    // INT8 slices carry their own power-of-two scale per row: any finite
+    // value is in range.
+    if (prec != Prec::F64 && prec != Prec::I8X4) {
         const size_t depth = naux + (signed_k ? naux_neg : 0);
         st = range_check(s, b_dev, in_f32, naux * nov, depth, stream);
         if (st != FH_OK) return st;
@@ -328,14 +339,16 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     pa.nvir = (int)nvir;
     pa.nvir_pad = pl.nvir_pad;
     pa.naux_pad = pl.naux_pad;
-    pa.fmt = prec == Prec::F64 ? PackFmt::F64
-             : prec == Prec::F32 ? PackFmt::F32
+    pa.fmt = prec == Prec::F64      ? PackFmt::F64
+             : prec == Prec::F32    ? PackFmt::F32
              : prec == Prec::BF16X3 ? PackFmt::BF16X2
+             : prec == Prec::I8X4   ? PackFmt::I8X4
                                     : PackFmt::FP8;
     pa.out0 = s.slots[kSlotOp0].ptr;
     pa.out1 = prec == Prec::BF16X3 ? s.slots[kSlotOp1].ptr : nullptr;
     pa.scale = prec == Prec::FP8 ? (uint8_t *)s.slots[kSlotScale].ptr : nullptr;
     pa.lo_sq = prec == Prec::BF16X3 ? (double *)s.slots[kSlotDiag].ptr : nullptr;
+    pa.rexp = prec == Prec::I8X4 ? (int *)s.slots[kSlotScale].ptr : nullptr;
     mark(s, 0, stream);
     hipError_t he;
     if (signed_k) {
@@ -364,6 +377,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     ea.op1 = pa.out1;
     ea.scale = pa.scale;
     ea.diag_corr = pa.lo_sq;
+    ea.rexp = pa.rexp;
     ea.nocc = (int)nocc;
     ea.nvir = (int)nvir;
     ea.nvir_pad = pl.nvir_pad;
@@ -415,6 +429,7 @@ EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec) {
     p.tile = choose_tile(nvir, prec);
     const int tuned = tuned_tile(prec, nvir, naux);
     if (tuned > 0) p.tile = tuned;
+    if (prec == Prec::I8X4 && p.tile > 64) p.tile = 64;  // 64 and 32 are built
     p.nvir_pad = (int)((nvir + 15) / 16 * 16);
     const int kp = std::max(panel_depth(prec), prec == Prec::FP8 ? 128 : 0);
     p.naux_pad = (int)((naux + kp - 1) / kp * kp);
@@ -435,6 +450,7 @@ void commit(State &s, Prec prec, bool adjoint, bool batched) {
     case Prec::F32: s.stats.fp32_calls++; break;
     case Prec::BF16X3: s.stats.bf16x3_calls++; break;
     case Prec::FP8: s.stats.fp8_calls++; break;
+    case Prec::I8X4: s.stats.int8x4_calls++; break;
     }
     if (adjoint) s.stats.adjoint_calls++;
     if (batched) s.stats.batched_calls++;
@@ -478,6 +494,7 @@ bool to_prec(int p, Prec *out) {
     case FH_FP32: *out = Prec::F32; return true;
     case FH_BF16X3: *out = Prec::BF16X3; return true;
     case FH_FP8_E4M3: *out = Prec::FP8; return true;
+    case FH_INT8X4: *out = Prec::I8X4; return true;
     default: return false;
     }
 }

@@ -33,6 +33,8 @@ enum SlotId {
     kSlotAcc,        // adjoint: dE/deps accumulators
     kSlotSplit,      // adjoint: split-K partial tiles of phase B
     kSlotRed,        // adjoint: per-row reduction partials
+    kSlotDressX,     // dressing: (Q|nu i) half-transformed rows
+    kSlotDressY,     // dressing: undressed MO rows (Q|ia)
     kNumSlots
 };
 
@@ -57,6 +59,7 @@ struct State {
     fh_stats stats{};
     Pending pending;
     GraphCache *graphs = nullptr;
+    void *blas = nullptr;  // rocBLAS handle of the device (dressing.hip), made on first use
     // Phase timing (FERRIC_TIMING): events on the call's stream.
     int timing = -1;  // -1: not yet read from the environment
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

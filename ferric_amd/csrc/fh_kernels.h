@@ -54,9 +54,19 @@ __host__ __device__ inline int fp8_scale_exponent(float amax) {
     return e;
 }
 
+// ---- INT8 digit slices -----------------------------------------------------
+
+// Row exponent of a packed row holding a non-finite value (its digits are 0
+// and its energy terms come out NaN).
+constexpr int kI8NonFinite = 1 << 20;
+// Largest naux_pad whose level sums fit int32: with |d1| <= 127 and the
+// other digits in [-128, 127], one level sums at most
+// 2 * 127 * 128 + 2 * 128 * 128 = 65280 per Q, and 65280 * 32832 < 2^31.
+constexpr int kI8MaxNauxPad = 32832;
+
 // ---- Packing ---------------------------------------------------------------
 
-enum class PackFmt { F64, F32, BF16X2, FP8 };
+enum class PackFmt { F64, F32, BF16X2, FP8, I8X4 };
 
 struct PackArgs {
     const void *b;    // caller layout, row-major (naux, nocc*nvir)
@@ -67,6 +77,7 @@ struct PackArgs {
     void *out1;       // BF16 lo
     uint8_t *scale;   // FP8: (nocc, naux_pad/64) E8M0 bytes
     double *lo_sq;    // BF16: per packed row, sum over Q of lo^2 (may be null)
+    int *rexp = nullptr;  // I8X4: per packed row, the exponent e of its scale 2^(e - 30)
     // F64/F32 only: row stride of the packed rows when this call fills one
     // segment of them (the two factors of a signed kernel); 0 = naux_pad. The
     // segment is naux_pad columns wide and out0 points at its first column.
@@ -79,7 +90,7 @@ hipError_t launch_absmax(const void *b, bool in_f32, int64_t n, unsigned int *ou
 
 // ---- Energy ----------------------------------------------------------------
 
-enum class Prec { F64 = 0, F32 = 1, BF16X3 = 2, FP8 = 3 };
+enum class Prec { F64 = 0, F32 = 1, BF16X3 = 2, FP8 = 3, I8X4 = 4 };
 
 // Panel depth along Q of each precision's kernel (naux_pad is a multiple).
 inline int panel_depth(Prec p) {
@@ -88,6 +99,7 @@ inline int panel_depth(Prec p) {
     case Prec::F32: return 16;
     case Prec::BF16X3: return 32;
     case Prec::FP8: return 128;
+    case Prec::I8X4: return 64;
     }
     return 128;
 }
@@ -99,6 +111,7 @@ struct EnergyArgs {
     const void *op1;          // BF16 lo
     const uint8_t *scale;     // FP8 scales (nocc, nqb)
     const double *diag_corr;  // BF16: sum_Q lo^2 per packed row, added to K_ii[a, a]
+    const int *rexp = nullptr;  // I8X4: per packed row, the exponent of its digit scale
     int nocc, nvir, nvir_pad, naux_pad, nqb;
     const double *eps_occ;    // device, active occupied (nocc)
     const double *eps_vir;    // device, virtual (nvir)
@@ -152,6 +165,7 @@ struct GemmArgs {
 };
 hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st);
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
+hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st);
 // Opposite-spin rectangle (F64 / F32) and its row reduction + fold.
 hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_rect_reduce(const EnergyArgs &a, hipStream_t st);

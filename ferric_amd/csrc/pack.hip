@@ -14,9 +14,12 @@
 // rows: the FP64 / FP32 packers then write with a row stride ld wider than the
 // segment, starting at the segment's first column.
 //
-// Formats: FP64, FP32, BF16 hi/lo split (hi = RN(x), lo = RN(x - hi)), and
+// Formats: FP64, FP32, BF16 hi/lo split (hi = RN(x), lo = RN(x - hi)),
 // OCP FP8 E4M3 with one power-of-two (E8M0) scale per (occupied i, 64-row Q
-// block).
+// block), and INT8 digit slices: per packed row one exponent f with
+// max_Q |x| <= (1 - 2^-7) 2^(f + 1), and per element the four signed byte
+// digits of Y = RN(x 2^(30 - f)); each 64-deep Q panel of a row is 256 bytes,
+// digit by digit (energy_i8.hip).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -242,6 +245,99 @@ __global__ __launch_bounds__(256) void absmax_kernel(const Tin *__restrict__ b, 
 
 __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
+// INT8 slices, step 1: the row exponent f of every packed row (i, a). With E
+// the frexp exponent of m = max_Q |B^Q_ia| (m < 2^E), f = E - 1 when
+// m <= (1 - 2^-7) 2^E and f = E otherwise, so that |Y| = |RN(x 2^(30 - f))| stays
+// at most 2^31 - 2^24 (top digit in [-127, 127]) and uses 31 bits for most rows;
+// 0 for a zero or padding row, kI8NonFinite for a row holding a NaN or an
+// infinity.
+// Block = 64 columns x 4 Q lanes (reads coalesced along a), grid
+// (ceil(nvir_pad / 64), nocc).
+template <typename Tin>
+__global__ __launch_bounds__(256) void i8_rowexp_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                        int naux, int nvir, int nvir_pad,
+                                                        int *__restrict__ rexp) {
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int a = blockIdx.x * 64 + tx, i = blockIdx.y;
+    double m = 0.0;
+    int bad = 0;
+    if (a < nvir) {
+        const Tin *p = b + (int64_t)i * nvir + a;
+        for (int q = ty; q < naux; q += 4) {
+            const double v = fabs((double)p[(int64_t)q * nov]);
+            if (v <= 1.7976931348623157e308)
+                m = fmax(m, v);
+            else
+                bad = 1;  // NaN or infinity
+        }
+    }
+    __shared__ double red[4][64];
+    __shared__ int flag[4][64];
+    red[ty][tx] = m;
+    flag[ty][tx] = bad;
+    __syncthreads();
+    if (ty == 0 && a < nvir_pad) {
+        const double mm = fmax(fmax(red[0][tx], red[1][tx]), fmax(red[2][tx], red[3][tx]));
+        int e = 0;
+        if (flag[0][tx] | flag[1][tx] | flag[2][tx] | flag[3][tx])
+            e = kI8NonFinite;
+        else if (mm > 0.0) {
+            (void)frexp(mm, &e);
+            if (mm <= ldexp(1.0 - 0x1p-7, e)) e -= 1;
+        }
+        rexp[(int64_t)i * nvir_pad + a] = e;
+    }
+}
+
+// INT8 slices, step 2: one 64x64 tile (64 Q = one panel, 64 packed rows).
+// Y = RN(x 2^(30 - f)) is exact arithmetic on the FP64 value
+// (|Y| <= 2^31 - 2^24); its digits are peeled from the bottom as signed bytes,
+// so d4, d3, d2 lie in [-128, 127] and the remaining top digit d1 in
+// [-127, 127].
+template <typename Tin>
+__global__ __launch_bounds__(256) void pack_i8x4_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                        int naux, int nvir, int nvir_pad,
+                                                        int64_t nrows, int naux_pad,
+                                                        const int *__restrict__ rexp,
+                                                        int8_t *__restrict__ out) {
+    __shared__ double tile[PT][PT + 1];
+    const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;
+    const int64_t r0 = (int64_t)blockIdx.x * PT;
+    const int q0 = blockIdx.y * PT;
+    const int64_t r = r0 + tx;
+    int64_t col = -1;
+    if (r < nrows) {
+        const int64_t i = r / nvir_pad;
+        const int a = (int)(r % nvir_pad);
+        if (a < nvir) col = i * nvir + a;
+    }
+    for (int q = ty; q < PT; q += 4) {
+        const int qq = q0 + q;
+        double v = 0.0;
+        if (col >= 0 && qq < naux) v = (double)b[(int64_t)qq * nov + col];
+        tile[q][tx] = v;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < PT; rr += 4) {
+        const int64_t orow = r0 + rr;
+        if (orow >= nrows) continue;
+        const int e = rexp[orow];
+        int y = 0;
+        if (e != kI8NonFinite) y = (int)rint(ldexp(tile[tx][rr], 30 - e));
+        const int d4 = (int)(int8_t)(y & 0xff);
+        y = (y - d4) >> 8;
+        const int d3 = (int)(int8_t)(y & 0xff);
+        y = (y - d3) >> 8;
+        const int d2 = (int)(int8_t)(y & 0xff);
+        const int d1 = (y - d2) >> 8;
+        int8_t *o = out + orow * ((int64_t)naux_pad * 4) + (int64_t)q0 * 4 + tx;
+        o[0] = (int8_t)d1;
+        o[64] = (int8_t)d2;
+        o[128] = (int8_t)d3;
+        o[192] = (int8_t)d4;
+    }
+}
+
 }  // namespace
 
 static inline dim3 pack_grid(int64_t nrows, int naux_pad) {
@@ -288,6 +384,18 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t st) {
             hipLaunchKernelGGL(fp8_scale_kernel<double>, sg, dim3(256), 0, st,
                                (const double *)a.b, nov, a.naux, a.nvir, nqb, a.scale);
         FH_LAUNCH(pack_fp8_kernel, nqb, (const uint8_t *)a.scale, (uint8_t *)a.out0);
+        break;
+    }
+    case PackFmt::I8X4: {
+        if (!a.rexp || a.naux_pad % PT != 0) return hipErrorInvalidValue;
+        dim3 eg((unsigned)((a.nvir_pad + 63) / 64), (unsigned)a.nocc);
+        if (f32in)
+            hipLaunchKernelGGL(i8_rowexp_kernel<float>, eg, dim3(256), 0, st, (const float *)a.b,
+                               nov, a.naux, a.nvir, a.nvir_pad, a.rexp);
+        else
+            hipLaunchKernelGGL(i8_rowexp_kernel<double>, eg, dim3(256), 0, st,
+                               (const double *)a.b, nov, a.naux, a.nvir, a.nvir_pad, a.rexp);
+        FH_LAUNCH(pack_i8x4_kernel, (const int *)a.rexp, (int8_t *)a.out0);
         break;
     }
     }

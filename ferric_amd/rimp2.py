@@ -60,13 +60,14 @@ _PREC_NAMES = {
     "f32": Precision.FP32, "fp32": Precision.FP32, "mixed": Precision.FP32,
     "bf16x3": Precision.BF16X3, "bf16": Precision.BF16X3,
     "fp8": Precision.FP8_E4M3, "fp8_e4m3": Precision.FP8_E4M3,
+    "int8x4": Precision.INT8X4, "int8": Precision.INT8X4,
 }
 
 
 def settings_precision() -> Precision:
     """Reduced precision allowed for the energy kernel by the process settings.
 
-    ``FERRIC_GPU_PRECISION`` (f64 | mixed | fp32 | bf16x3 | fp8), the reference's
+    ``FERRIC_GPU_PRECISION`` (f64 | mixed | fp32 | bf16x3 | fp8 | int8x4), the reference's
     precision knob (crates/ferric-core/src/gpu/precision.rs:283); default f64.
     A malformed value is an error, as with the reference's ConfigVar.
     """

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 3
+#define FERRIC_HIP_ABI_VERSION 4
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -63,6 +63,12 @@ typedef enum fh_status {
  *   FH_BF16X3   BF16 hi/lo split, hi*hi + hi*lo + lo*hi, FP32 accumulation
  *   FH_FP8_E4M3 OCP E4M3 operands with one power-of-two scale per
  *               (64-row Q block, occupied i), FP32 accumulation
+ *   FH_INT8X4   every element as four signed byte digits of Y = RN(x 2^(30-e)),
+ *               |Y| < 2^31, e per (i, a) row from max_Q |B^Q_ia|; the 13 digit
+ *               products of levels 2..6 accumulated exactly in int32 (INT8
+ *               MFMA) and combined in FP64: only the rounding of Y (2^-32 of
+ *               the row's power-of-two range) and the dropped levels depart
+ *               from FP64. naux up to 32832 (FH_RANGE beyond).
  * The reference's Precision::{F64, Mixed} (crates/ferric-core/src/gpu/precision.rs)
  * map to FH_FP64 and FH_FP32. */
 typedef enum fh_precision {
@@ -70,6 +76,7 @@ typedef enum fh_precision {
     FH_FP32 = 1,
     FH_BF16X3 = 2,
     FH_FP8_E4M3 = 3,
+    FH_INT8X4 = 4,
 } fh_precision;
 
 /* Element type of a device-resident b_ov. */
@@ -106,6 +113,7 @@ typedef struct fh_stats {
     uint64_t fail_range;
     uint64_t fail_unsupported;
     uint64_t pool_peak_bytes;  /* high-water mark of the library's workspace */
+    uint64_t int8x4_calls;
 } fh_stats;
 
 /* One local-MP2 pair domain for fh_rimp2_batched_dev: its B (row-major
@@ -232,6 +240,22 @@ fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom, const doub
                                size_t b_len, const double *eps_dev, size_t eps_len,
                                fh_precision precision, int use_graph, void *stream,
                                double *e_os, double *e_ss);
+
+/* ---- Dressing / MO transform upstream of the pair stage ---------------- */
+
+/* stream_dressed_mo_band (crates/ferric-mp2/src/rimp2.rs:648-756) on the
+ * device: the RI factor of the MO pairs (l, r) over an output band of aux rows,
+ *   B[P, l*nright + r] = sum_Q m[P, Q] sum_{mu,nu} c_left[mu, l] (Q|mu nu) c_right[nu, r],
+ * ready for fh_rimp2_energy_dev (c_left: active occupied, c_right: virtual
+ * coefficients), so B never crosses PCIe. Device pointers, FP64, row-major:
+ * j3 (naux, nao, nao), symmetric in (mu, nu); c_left (nao, nleft); c_right
+ * (nao, nright); m (nband, naux), the band's rows of the dressing matrix
+ * (V^-1/2 in the reference, or L^-1); b_out (nband, nleft*nright). The two
+ * intermediates live in the library's workspace; the three GEMMs run in
+ * rocBLAS on the given stream. */
+fh_status fh_dress_b_ov_dev(const double *j3, size_t naux, size_t nao, const double *c_left,
+                            size_t nleft, const double *c_right, size_t nright, const double *m,
+                            size_t nband, void *stream, double *b_out);
 
 /* ---- Unrestricted (open-shell) blocks and the signed factor ------------ */
 

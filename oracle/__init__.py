@@ -71,6 +71,7 @@ def lib():
                                           _dp, _sz, _sz, _sz, ctypes.c_int, _dp]
         L.orc_signed_spin_components.argtypes = [_dp, _sz, _dp, _sz, _sz, _sz, _dp, _sz, _sz,
                                                  _sz, ctypes.c_int, _dp]
+        L.orc_dress_b_ov.argtypes = [_dp, _sz, _sz, _dp, _sz, _dp, _sz, _dp, _sz, _dp]
         _lib = L
     return _lib
 
@@ -296,3 +297,21 @@ def signed_spin_components(pos, neg, eps, nocc, nvir, first_occ=0, nocc_total=No
     if rc:
         raise OracleError(f"oracle error {rc}")
     return tuple(float(x) for x in out)
+
+
+def dress_b_ov(j3, v_inv_sqrt, c_left, c_right, output_band=None):
+    """B = M (C_left^T (Q|mu nu) C_right) of the output band (rimp2.rs:648-756).
+
+    j3 (naux, nao, nao); v_inv_sqrt (naux, naux); c_left (nao, nleft);
+    c_right (nao, nright). Returns (band, nleft*nright)."""
+    a = _f64(j3)
+    naux, nao = a.shape[0], a.shape[1]
+    p0, p1 = output_band if output_band is not None else (0, naux)
+    m = _f64(np.asarray(v_inv_sqrt)[p0:p1])
+    cl, cr = _f64(c_left), _f64(c_right)
+    out = np.zeros((p1 - p0, cl.shape[1] * cr.shape[1]))
+    rc = lib().orc_dress_b_ov(_p(a), naux, nao, _p(cl), cl.shape[1], _p(cr), cr.shape[1], _p(m),
+                              p1 - p0, _p(out))
+    if rc:
+        raise OracleError(f"oracle error {rc}")
+    return out

@@ -27,6 +27,7 @@
  *                                          crates/ferric-mp2/src/u_rimp2.rs:731-760, 978-995
  *   - signed_pair_block / spin_components_from_signed_b_ov
  *                                          crates/ferric-mp2/src/rimp2.rs:1559-1607
+ *   - stream_dressed_mo_band(_budgeted)    crates/ferric-mp2/src/rimp2.rs:648-756
  *
  * The reference forms G_i with an OpenBLAS dgemm; here it is a plain blocked
  * loop in the same (P-outer) association class. The two agree to the
@@ -654,4 +655,54 @@ int orc_signed_spin_components(const double *pos, size_t naux_p, const double *n
     out[1] = ss;
     out[2] = os + ss;
     return failed ? ORC_ALLOC : ORC_OK;
+}
+
+/* stream_dressed_mo_band_budgeted (rimp2.rs:666-756) with no budget: per aux
+ * row Q, half = C_left^T (Q|mu nu) (nleft x nao), mo = half C_right
+ * (nleft x nright); chunks of 256 rows of Q dressed into the output band with
+ * beta = 1 accumulation, out[P, :] += m[P, Q-chunk] . mo-chunk. j3 is
+ * (naux, nao, nao) row-major, c_left (nao, nleft), c_right (nao, nright), m
+ * (nband, naux) -- the band's rows of the dressing matrix -- and out
+ * (nband, nleft*nright), all row-major. */
+int orc_dress_b_ov(const double *j3, size_t naux, size_t nao, const double *c_left, size_t nleft,
+                   const double *c_right, size_t nright, const double *m, size_t nband,
+                   double *out) {
+    const size_t width = nleft * nright, chunk = 256;
+    double *half = (double *)malloc(sizeof(double) * nleft * nao);
+    double *mo = (double *)malloc(sizeof(double) * chunk * width);
+    if (!half || !mo) {
+        free(half);
+        free(mo);
+        return ORC_ALLOC;
+    }
+    memset(out, 0, sizeof(double) * nband * width);
+    for (size_t q0 = 0; q0 < naux; q0 += chunk) {
+        const size_t q1 = q0 + chunk < naux ? q0 + chunk : naux;
+        for (size_t q = q0; q < q1; ++q) {
+            const double *a = j3 + q * nao * nao;
+            for (size_t l = 0; l < nleft; ++l)
+                for (size_t nu = 0; nu < nao; ++nu) {
+                    double s = 0.0;
+                    for (size_t mu = 0; mu < nao; ++mu) s += c_left[mu * nleft + l] * a[mu * nao + nu];
+                    half[l * nao + nu] = s;
+                }
+            double *row = mo + (q - q0) * width;
+            for (size_t l = 0; l < nleft; ++l)
+                for (size_t r = 0; r < nright; ++r) {
+                    double s = 0.0;
+                    for (size_t nu = 0; nu < nao; ++nu) s += half[l * nao + nu] * c_right[nu * nright + r];
+                    row[l * nright + r] = s;
+                }
+        }
+        for (size_t p = 0; p < nband; ++p)
+            for (size_t q = q0; q < q1; ++q) {
+                const double w = m[p * naux + q];
+                const double *row = mo + (q - q0) * width;
+                double *o = out + p * width;
+                for (size_t c = 0; c < width; ++c) o[c] += w * row[c];
+            }
+    }
+    free(half);
+    free(mo);
+    return ORC_OK;
 }

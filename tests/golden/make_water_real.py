@@ -19,7 +19,10 @@ Method (this file's own code):
   * RHF: core-Hamiltonian guess, DIIS, converged to 1e-12 Eh and 1e-10 in the
     orbital gradient;
   * RI: (P|Q) = L L^T (Cholesky), B = L^-1 (P|ia) -- the reference's dressing
-    (rimp2.rs SignedBOv doc, "dressed by the Cholesky factor").
+    (rimp2.rs SignedBOv doc, "dressed by the Cholesky factor"). For the
+    distorted geometry the fixture also keeps (Q|mu nu), the MO coefficients
+    and (P|Q), the inputs of the dressing / MO-transform stage
+    (stream_dressed_mo_band, rimp2.rs:648-756).
 
 Two geometries: the equilibrium water of testdata/molecules/water.xyz (known
 answer testdata/reference/h2o_cc-pvdz_rimp2.json) and the distorted water of
@@ -452,7 +455,7 @@ def build(xyz_path):
     pia = np.einsum("Pmn,mi,na->Pia", j3, co, cv).reshape(naux, nocc * nvir)
     b = np.linalg.solve(np.linalg.cholesky(j2), pia)
     return dict(b_ov=b, eps=e_orb, nocc=nocc, nvir=nvir, naux=naux, e_rhf=e_rhf, enuc=enuc,
-                iterations=it + 1)
+                iterations=it + 1, j3=j3, c=C, j2=j2)
 
 
 def numpy_mp2(b, eps, nocc, nvir, fc=0):
@@ -506,7 +509,9 @@ def main():
                dist_e_corr_fc_orca=grad["orca_frozen_core"]["e_corr"],
                dist_e_os_ref=scs["pyscf"]["e_os"], dist_e_ss_ref=scs["pyscf"]["e_ss"],
                dist_e_os_fc_ref=scs["pyscf"]["e_os_fc"], dist_e_ss_fc_ref=scs["pyscf"]["e_ss_fc"],
-               dist_e_rhf_ref=grad["pyscf"]["e_rhf"])
+               dist_e_rhf_ref=grad["pyscf"]["e_rhf"],
+               # inputs of the dressing / MO transform: (Q|mu nu), MO coefficients, (P|Q)
+               dist_j3=d["j3"], dist_c=d["c"], dist_j2=d["j2"])
     np.savez_compressed(OUT, **out)
     print("wrote", OUT)
 

@@ -9,6 +9,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int v4i __attribute__((ext_vector_type(4)));
 
 __global__ void k_f64(const double *a, const double *b, double *c) {
     const int l = threadIdx.x;
@@ -45,9 +46,23 @@ __global__ void k_f8(const int *a, const int *b, const int *sa, const int *sb, f
     for (int v = 0; v < 4; ++v) c[4 * l + v] = acc[v];
 }
 
+__global__ void k_i8(const int *a, const int *b, int *c) {
+    const int l = threadIdx.x;
+    v4i x, y;
+    for (int e = 0; e < 4; ++e) {
+        x[e] = a[4 * l + e];
+        y[e] = b[4 * l + e];
+    }
+    v4i acc = {0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(x, y, acc, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) c[4 * l + v] = acc[v];
+}
+
+// kind 0 = f64 16x16x4, 1 = f32 16x16x4, 2 = bf16 16x16x32, 3 = scaled f8
+// 16x16x128, 4 = i8 16x16x64 (16 bytes per lane, int32 result).
 extern "C" int probe_mfma(int kind, const void *a, const void *b, const int *sa, const int *sb,
                           void *c) {
-    size_t ab = kind == 0 ? 64 * 8 : kind == 1 ? 64 * 4 : kind == 2 ? 64 * 16 : 64 * 32;
+    size_t ab = kind == 0 ? 64 * 8 : kind == 1 ? 64 * 4 : kind == 2 || kind == 4 ? 64 * 16 : 64 * 32;
     size_t cb = kind == 0 ? 64 * 4 * 8 : 64 * 4 * 4;
     void *da, *db, *dc;
     int *dsa = nullptr, *dsb = nullptr;
@@ -65,6 +80,7 @@ extern "C" int probe_mfma(int kind, const void *a, const void *b, const int *sa,
     case 1: hipLaunchKernelGGL(k_f32, dim3(1), dim3(64), 0, 0, (const float *)da, (const float *)db, (float *)dc); break;
     case 2: hipLaunchKernelGGL(k_bf16, dim3(1), dim3(64), 0, 0, (const uint16_t *)da, (const uint16_t *)db, (float *)dc); break;
     case 3: hipLaunchKernelGGL(k_f8, dim3(1), dim3(64), 0, 0, (const int *)da, (const int *)db, (const int *)dsa, (const int *)dsb, (float *)dc); break;
+    case 4: hipLaunchKernelGGL(k_i8, dim3(1), dim3(64), 0, 0, (const int *)da, (const int *)db, (int *)dc); break;
     default: return 4;
     }
     if (hipGetLastError() != hipSuccess) return 5;
@@ -138,6 +154,23 @@ __global__ __launch_bounds__(256) void rate_f8(int iters, double *out) {
         for (int k = 0; k < kRateAcc; ++k)
             acc[k] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(x, y, acc[k], 0, 0, 0, 127,
                                                                         0, 127);
+    double s = 0;
+    for (int k = 0; k < kRateAcc; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void rate_i8(int iters, double *out) {
+    v4i x, y;
+    for (int e = 0; e < 4; ++e) {
+        x[e] = 0x01020304 + (int)threadIdx.x;
+        y[e] = 0x05060708 - e;
+    }
+    v4i acc[kRateAcc];
+    for (int k = 0; k < kRateAcc; ++k) acc[k] = (v4i){0, 0, 0, 0};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < kRateAcc; ++k)
+            acc[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(x, y, acc[k], 0, 0, 0);
     double s = 0;
     for (int k = 0; k < kRateAcc; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
     out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
@@ -253,7 +286,7 @@ __global__ __launch_bounds__(256) void rate_lds_fma_il(int iters, double *out) {
 // with 8 / 4 accumulators, 7 = 8 f64 MFMA + 16 vector FMA chains, 8 = 16 f64
 // MFMA + 32 vector FMA chains, 9 / 10 / 11 = LDS-fed vector FMA with 8x12 /
 // 8x8 / 6x8 register tiles, 12 / 13 = 8x8 interleaved layout at one / three
-// workgroups per CU. Returns the
+// workgroups per CU, 14 = i8 16x16x64 (int32 accumulation). Returns the
 // FLOP rate in TFLOP/s of the second of two identical launches in *tflops.
 extern "C" int probe_mfma_rate(int kind, int blocks, int iters, double *tflops) {
     if (blocks <= 0 || iters <= 0) return 4;
@@ -274,6 +307,7 @@ extern "C" int probe_mfma_rate(int kind, int blocks, int iters, double *tflops) 
                                  : kind == 9 ? vf64 * kLdsKP * 8 * 12
                                  : kind == 10 ? vf64 * kLdsKP * 8 * 8
                                  : kind == 11 ? vf64 * kLdsKP * 6 * 8
+                                 : kind == 14 ? 2.0 * 16 * 16 * 64 * kRateAcc
                                               : vf64 * kLdsKP * 8 * 8;
     float ms = 0;
     for (int rep = 0; rep < 2; ++rep) {
@@ -291,6 +325,7 @@ extern "C" int probe_mfma_rate(int kind, int blocks, int iters, double *tflops) 
         case 13: hipLaunchKernelGGL((rate_lds_fma_il<1>), dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 2: hipLaunchKernelGGL(rate_bf16, dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 3: hipLaunchKernelGGL(rate_f8, dim3(blocks), dim3(256), 0, 0, iters, out); break;
+        case 14: hipLaunchKernelGGL(rate_i8, dim3(blocks), dim3(256), 0, 0, iters, out); break;
         case 4: hipLaunchKernelGGL(rate_vfma_f64, dim3(blocks), dim3(256), 0, 0, iters, out); break;
         default: return 4;
         }

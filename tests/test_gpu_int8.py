@@ -1,0 +1,107 @@
+"""The INT8 digit-slice kernel (Precision.INT8X4) against the oracle, run with -m gpu.
+
+* Exact format: on the golden fixtures (tile-32 and tile-64 shapes, frozen
+  core, single-product and two-product items) the GPU energy equals the oracle
+  energy (rimp2.rs:984-1018 restated) of quant.int8x4_g(b), the g the format
+  forms bit for bit, up to FP64 summation order. This pins the digit packer,
+  the lane layout of the int32 MFMA result and the level combination.
+* Against FP64: the golden cases and the decane-scale workload within the
+  format's derived bound (quant.int8x4_energy_bound: half a unit of rounding
+  per element and the dropped levels, propagated through the pair sums), and
+  the 100/900/3600 workload within 1 uEh of the FP64 kernel.
+* Errors: naux beyond the int32 level-sum range is a RangeError (after which the
+  dispatcher's FP64 retry returns the FP64 energy); a NaN in B gives NaN.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import ferric_amd  # noqa: E402
+from ferric_amd import Precision, rimp2, synthetic  # noqa: E402
+from tests import quant  # noqa: E402
+
+I8 = Precision.INT8X4
+
+
+def _gpu(b, eps, nocc, nvir, fc=0, nt=None, kappa=None, prec=I8):
+    t = torch.from_numpy(np.ascontiguousarray(b)).cuda()
+    return rimp2.spin_components(t, eps, nocc, nvir, fc, nt, kappa, prec)
+
+
+def test_exact_format_golden(golden, oracle_mod):
+    for name, c in golden.items():
+        print("int8x4 exact format", name, flush=True)
+        nocc, nvir, fc, nt = c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"]
+        b, eps = c["b"], c["eps"]
+        sc = _gpu(b, eps, nocc, nvir, fc, nt)
+        want = oracle_mod.spin_components_from_g(quant.int8x4_g(b), eps, nocc, nvir, fc, nt)
+        tol = 1e-13 * (abs(want[0]) + abs(want[1])) + 1e-15
+        assert abs(sc.e_os - want[0]) <= tol, (name, sc, want)
+        assert abs(sc.e_ss - want[1]) <= tol, (name, sc, want)
+        # and against the FP64 energy of the fixture (its generator's brute force)
+        d_os, d_ss = sc.e_os - c["energy"][0], sc.e_ss - c["energy"][1]
+        bos, bss = quant.int8x4_energy_bound(b, eps, nocc, nvir, fc, nt)
+        print(f"  dE vs fp64 {d_os:.2e} {d_ss:.2e}  bound {bos:.2e} {bss:.2e}", flush=True)
+        assert abs(d_os) <= bos + 1e-14 and abs(d_ss) <= bss + 1e-14, name
+        sk = _gpu(b, eps, nocc, nvir, fc, nt, kappa=1.3)
+        kos, kss = c["energy_kappa"]
+        assert abs(sk.e_os - kos) <= bos + 1e-14 and abs(sk.e_ss - kss) <= bss + 1e-14, name
+
+
+def test_decane_scale_within_the_format_bound(oracle_mod):
+    p = synthetic.decane_scale()
+    print("int8x4 decane-scale", flush=True)
+    ref = oracle_mod.spin_components_blas(p.b_ov, p.eps, p.nocc, p.nvir, nthreads=16)
+    sc = _gpu(p.b_ov, p.eps, p.nocc, p.nvir)
+    bos, bss = quant.int8x4_energy_bound(p.b_ov, p.eps, p.nocc, p.nvir)
+    print(f"  E_os {sc.e_os:.13f} ({sc.e_os - ref[0]:.2e}, bound {bos:.2e})  E_ss "
+          f"{sc.e_ss:.13f} ({sc.e_ss - ref[1]:.2e}, bound {bss:.2e})", flush=True)
+    assert abs(sc.e_os - ref[0]) <= bos and abs(sc.e_ss - ref[1]) <= bss
+    # the error is FP64-class: far inside 1 uEh
+    assert abs(sc.e_total - ref[2]) <= 1e-8
+    # FP32 storage of the same values goes through the same digits
+    s32 = _gpu(p.b_ov.astype(np.float32), p.eps, p.nocc, p.nvir)
+    s64 = _gpu(p.b_ov.astype(np.float32).astype(np.float64), p.eps, p.nocc, p.nvir)
+    assert (s32.e_os, s32.e_ss) == (s64.e_os, s64.e_ss)
+
+
+def test_mixed_workload_within_one_micro_hartree():
+    p = synthetic.mixed_large()
+    print("int8x4 mixed 100/900/3600", flush=True)
+    t = torch.from_numpy(p.b_ov).cuda()
+    f64 = rimp2.spin_components(t, p.eps, p.nocc, p.nvir, precision=Precision.FP64)
+    i8 = rimp2.spin_components(t, p.eps, p.nocc, p.nvir, precision=I8)
+    d = i8.e_total - f64.e_total
+    print(f"  E fp64 {f64.e_total:.9f} int8x4 {i8.e_total:.9f} dE {d:.3e}", flush=True)
+    assert abs(d) <= 1e-6
+    del t
+    torch.cuda.empty_cache()
+
+
+def test_range_and_non_finite():
+    rng = np.random.default_rng(7)
+    nocc, nvir = 2, 6
+    eps = np.concatenate([[-1.0, -0.8], np.linspace(0.3, 1.5, nvir)])
+    big = rng.standard_normal((32900, nocc * nvir)) * 1e-2
+    print("int8x4 range", flush=True)
+    with pytest.raises(ferric_amd.RangeError):
+        _gpu(big, eps, nocc, nvir)
+    ref = _gpu(big, eps, nocc, nvir, prec=Precision.FP64)
+    tr = rimp2.try_spin_components_on_device(big, eps, nocc, nvir, 0, nocc, mixed_ok=True)
+    assert tr is not None
+    os.environ["FERRIC_GPU_PRECISION"] = "int8x4"
+    try:
+        tr = rimp2.try_spin_components_on_device(big, eps, nocc, nvir, 0, nocc, mixed_ok=True)
+    finally:
+        del os.environ["FERRIC_GPU_PRECISION"]
+    assert (tr.e_os, tr.e_ss) == (ref.e_os, ref.e_ss)
+    b = rng.standard_normal((100, nocc * nvir)) * 0.03
+    b[17, 4] = np.nan
+    print("int8x4 nan", flush=True)
+    sc = _gpu(b, eps, nocc, nvir)
+    assert np.isnan(sc.e_total)

@@ -4,6 +4,8 @@ Assumed (and relied on by ferric_amd/csrc/energy*.hip):
 * 16x16xK: lane l = 16 g + r supplies A[row r][k] and B[k][col r] for its k set
   (the same set for A and B), and result element v of lane l is C[4 g + v][r] (C[4 v + g][r] for
   v_mfma_f64_16x16x4f64);
+* 16x16x64 INT8 (int32 result): the same row 4 g + v as the FP32-accumulating
+  instructions, bytes paired between A and B lanes of one group g (exact);
 * scaled FP8 (format 0 = OCP E4M3, E8M0 bias 127): bytes 0-15 of lane group g
   are k = 16g..16g+15, bytes 16-31 are k = 64+16g..; the scale of 32-wide k
   block kb of row r comes from lane 16 kb + r.
@@ -159,3 +161,19 @@ def test_f8_scaled_16x16x128(probe, codeset):
     got = _to_layout(c)
     print("f8 scaled: got", got[0, :4], "want", want[0, :4])
     assert np.allclose(got, want, rtol=1e-5, atol=1e-3 * np.abs(want).max())
+
+
+def test_i8_16x16x64(probe):
+    """Byte e of lane 16 g + r of A meets byte e of lane 16 g + c of B (the
+    kernels feed both from the same Q positions), and result element v of lane
+    16 g + r is C[4 g + v][r], exactly in int32."""
+    rng = np.random.default_rng(5)
+    a = rng.integers(-128, 128, size=(64, 16)).astype(np.int8)
+    b = rng.integers(-128, 128, size=(64, 16)).astype(np.int8)
+    c = _run(probe, 4, a.view(np.int32), b.view(np.int32), np.zeros((64, 4), np.int32))
+    av = a.astype(np.int64).reshape(4, 16, 16)
+    bv = b.astype(np.int64).reshape(4, 16, 16)
+    want = np.einsum("gre,gce->rc", av, bv)
+    got = _to_layout(c.astype(np.float64))
+    print("i8 rows", _discover(c.astype(np.float64), want.astype(np.float64)))
+    assert np.array_equal(got, want.astype(np.float64))
