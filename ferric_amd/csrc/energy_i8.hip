@@ -26,6 +26,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fh_device.h"
 #include "fh_kernels.h"
 #include "pair_body.h"
@@ -75,37 +77,45 @@ __global__ __launch_bounds__(256) void energy_i8x4_kernel(EnergyArgs a) {
     const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
     const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
     const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
-    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
     const int64_t ldr = (int64_t)a.naux_pad * 4;  // bytes per packed row
     const int tid = threadIdx.x;
 
+    // Global -> LDS staging. Thread t owns chunk ch = t % 16 of rows
+    // r0 + 16 k (r0 = t / 16) of every operand, so its four row pointers are
+    // loop-invariant and every load is pointer + uniform offset. Rows past
+    // nvir_pad (a last tile narrower than TILE) are read from the next occupied
+    // index or from the buffer's TILE spare rows: they feed only the 16-row
+    // blocks that rv / cv switch off, so no row test is needed.
+    constexpr int RPS = 256 / CPR;  // rows per pass of the workgroup (16)
+    constexpr int SPO = TILE / RPS; // passes per operand
+    static_assert(CPT == 4 * SPO && NCH == 256 * CPT, "one chunk per thread and pass");
+    const int ch = tid % CPR, r0 = tid / CPR;
+    const int8_t *src[4];
+    src[0] = bp + (base0 + r0) * ldr + ch * 16;
+    src[1] = bp + (base1 + r0) * ldr + ch * 16;
+    src[2] = bp + (base2 + r0) * ldr + ch * 16;
+    src[3] = bp + (base3 + r0) * ldr + ch * 16;
     v4i stg[CPT];
     auto gload = [&](int q0) {
         const int q23 = q0 + sq.off23;
+        // uniform: a single-product item's extra panel past the end is zeros
+        const bool hi_ok = q23 < a.naux_pad;
 #pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            const int c = tid + 256 * t;
-            const int op = c / OPCH, rem = c % OPCH;
-            const int row = rem / CPR, ch = rem % CPR;
-            const int64_t base = op == 0 ? base0 : op == 1 ? base1 : op == 2 ? base2 : base3;
-            const int lim = (op & 1) ? limB : limA;
-            const int q = op < 2 ? q0 : q23;
-            v4i v = {0, 0, 0, 0};
-            if (c < NCH && row < lim && q < a.naux_pad)
-                v = *(const v4i *)(bp + (base + row) * ldr + (int64_t)q * 4 + ch * 16);
-            stg[t] = v;
-        }
+        for (int op = 0; op < 4; ++op)
+#pragma unroll
+            for (int k = 0; k < SPO; ++k) {
+                const int64_t off = (int64_t)k * RPS * ldr + (int64_t)(op < 2 ? q0 : q23) * 4;
+                v4i v = {0, 0, 0, 0};
+                if (op < 2 || hi_ok) v = *(const v4i *)(src[op] + off);
+                stg[op * SPO + k] = v;
+            }
     };
+    // (r0 + 16 k) & 15 == r0: one swizzled slot per thread.
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            const int c = tid + 256 * t;
-            if (c < NCH) {
-                const int op = c / OPCH, rem = c % OPCH;
-                const int row = rem / CPR, ch = rem % CPR;
-                lds[buf][op][row][ch ^ (row & 15)] = stg[t];
-            }
-        }
+        for (int op = 0; op < 4; ++op)
+#pragma unroll
+            for (int k = 0; k < SPO; ++k) lds[buf][op][r0 + RPS * k][ch ^ r0] = stg[op * SPO + k];
     };
 
     const int lane = tid % kWave, wid = tid / kWave;
@@ -132,7 +142,9 @@ __global__ __launch_bounds__(256) void energy_i8x4_kernel(EnergyArgs a) {
     auto frag = [&](int buf, int op, int row, int d) -> v4i {
         return lds[buf][op][row][(d * 4 + L.g) ^ (row & 15)];
     };
-    auto compute = [&](int buf) {
+    // FULL: every 16x16 block of the wave lies inside nvir_pad (the common
+    // case), so the MFMA sequence needs no per-block test.
+    auto compute = [&](int buf, auto full) {
         v4i f0[4][NB], f2[4][NB], f1[4][NB], f3[4][NB];
 #pragma unroll
         for (int d = 0; d < 4; ++d)
@@ -148,7 +160,7 @@ __global__ __launch_bounds__(256) void energy_i8x4_kernel(EnergyArgs a) {
         for (int x = 0; x < NB; ++x)
 #pragma unroll
             for (int y = 0; y < NB; ++y)
-                if (rv[x] && cv[y]) {
+                if (decltype(full)::value || (rv[x] && cv[y])) {
 #pragma unroll
                     for (int p = 0; p < 4; ++p)
 #pragma unroll
@@ -162,13 +174,20 @@ __global__ __launch_bounds__(256) void energy_i8x4_kernel(EnergyArgs a) {
                 }
     };
 
+    bool full = true;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
+
     gload(0);
     lstore(0);
     __syncthreads();
     for (int p = 0; p < sq.iters; ++p) {
         const int cur = p & 1;
         if (p + 1 < sq.iters) gload((p + 1) * sq.step);
-        compute(cur);
+        if (full)
+            compute(cur, std::true_type{});
+        else
+            compute(cur, std::false_type{});
         if (p + 1 < sq.iters) lstore(cur ^ 1);
         __syncthreads();
     }
@@ -221,12 +240,253 @@ __global__ __launch_bounds__(256) void energy_i8x4_kernel(EnergyArgs a) {
     pair::energy_epilogue<TILE>(a, it, L, g1, g2, rv, cv, pair::RowOf<float>{}, pair::NoCorr{});
 }
 
+// 64-wide tile on 8 waves: waves 0-3 accumulate C1 (operands 0 and 1), waves
+// 4-7 accumulate C2 (operands 2 and 3), each wave one 32x32 quadrant. Half the
+// accumulators per wave of the 4-wave kernel let two waves share each SIMD, so
+// one wave's LDS fragment reads overlap the other's MFMAs, and the registers
+// left over hold a second staged panel: global loads run two panels ahead of
+// the MFMAs. For a single-product item waves 4-7 take the odd panels of C1.
+// The halves meet through LDS after the loop (exact int32 merge of the level
+// sums for a single-product item; FP64 C2 for a two-product item), and waves
+// 0-3 run the energy epilogue.
+__global__ __launch_bounds__(512) void energy_i8x4_wide_kernel(EnergyArgs a) {
+    constexpr int TILE = 64, KP = 64;
+    constexpr int TW = 32, NB = 2;
+    constexpr int CPR = 16;             // 16-byte chunks per row and panel
+    constexpr int RPS = 512 / CPR;      // rows per pass of the workgroup (32)
+    constexpr int SPO = TILE / RPS;     // passes per operand (2)
+    constexpr int CPT = 4 * SPO;        // staged chunks per thread and panel (8)
+
+    __shared__ __attribute__((aligned(16))) v4i lds[2][4][TILE][CPR];
+
+    pair::Item it;
+    it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
+    decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
+    decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
+
+    const int8_t *__restrict__ bp = (const int8_t *)a.op0;
+    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
+    const pair::SplitQ sq(it, a.naux_pad, KP);
+    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
+    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
+    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
+    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
+    const int64_t ldr = (int64_t)a.naux_pad * 4;
+    const int tid = threadIdx.x;
+
+    // Staging as in the 4-wave kernel (rows past nvir_pad only feed switched-off
+    // blocks; the buffer has TILE spare rows): thread t owns chunk t % 16 of
+    // rows t / 16 + 32 k of every operand.
+    const int ch = tid % CPR, r0 = tid / CPR;
+    const int8_t *src[4];
+    src[0] = bp + (base0 + r0) * ldr + ch * 16;
+    src[1] = bp + (base1 + r0) * ldr + ch * 16;
+    src[2] = bp + (base2 + r0) * ldr + ch * 16;
+    src[3] = bp + (base3 + r0) * ldr + ch * 16;
+    v4i stg[2][CPT];
+    auto gload = [&](int p, v4i(&s)[CPT]) {
+        const int q0 = p * sq.step, q23 = q0 + sq.off23;
+        const bool hi_ok = q23 < a.naux_pad;  // uniform
+#pragma unroll
+        for (int op = 0; op < 4; ++op)
+#pragma unroll
+            for (int k = 0; k < SPO; ++k) {
+                const int64_t off = (int64_t)k * RPS * ldr + (int64_t)(op < 2 ? q0 : q23) * 4;
+                v4i v = {0, 0, 0, 0};
+                if (op < 2 || hi_ok) v = *(const v4i *)(src[op] + off);
+                s[op * SPO + k] = v;
+            }
+    };
+    auto lstore = [&](int buf, const v4i(&s)[CPT]) {
+#pragma unroll
+        for (int op = 0; op < 4; ++op)
+#pragma unroll
+            for (int k = 0; k < SPO; ++k)
+                lds[buf][op][r0 + RPS * k][ch ^ (r0 & 15)] = s[op * SPO + k];
+    };
+
+    const int lane = tid % kWave, wid = tid / kWave;
+    const int prod = wid >> 2, quad = wid & 3;
+    const pair::Lane L{A0, B0, quad >> 1, quad & 1, lane % 16, lane / 16};
+    const int opA = 2 * prod, opB = 2 * prod + 1;
+    bool rv[NB], cv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
+        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
+    }
+    v4i acc[kLevels][NB][NB];
+#pragma unroll
+    for (int l = 0; l < kLevels; ++l)
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) acc[l][x][y] = (v4i){0, 0, 0, 0};
+
+    auto compute = [&](int buf, auto full) {
+        v4i fa[4][NB], fb[4][NB];
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int ra = L.wr * TW + k * 16 + L.r, rb = L.wc * TW + k * 16 + L.r;
+                fa[d][k] = lds[buf][opA][ra][(d * 4 + L.g) ^ L.r];
+                fb[d][k] = lds[buf][opB][rb][(d * 4 + L.g) ^ L.r];
+            }
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y)
+                if (decltype(full)::value || (rv[x] && cv[y])) {
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (p + q < kLevels)
+                                acc[p + q][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                    fa[p][x], fb[q][y], acc[p + q][x][y], 0, 0, 0);
+                }
+    };
+    bool full = true;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
+    auto panel = [&](int buf) {
+        if (full)
+            compute(buf, std::true_type{});
+        else
+            compute(buf, std::false_type{});
+    };
+
+    // Two panels in flight: at iteration p, LDS buffer p & 1 holds panel p,
+    // stg[(p + 1) & 1] panel p + 1 (landing), and panel p + 2 is fetched into
+    // stg[p & 1]. The loop is unrolled by two so the staging index is static.
+    const int iters = sq.iters;
+    gload(0, stg[0]);
+    if (iters > 1) gload(1, stg[1]);
+    lstore(0, stg[0]);
+    __syncthreads();
+    for (int p = 0; p < iters; p += 2) {
+        if (p + 2 < iters) gload(p + 2, stg[0]);
+        panel(0);
+        if (p + 1 < iters) lstore(1, stg[1]);
+        __syncthreads();
+        if (p + 1 >= iters) break;
+        if (p + 3 < iters) gload(p + 3, stg[1]);
+        panel(1);
+        if (p + 2 < iters) lstore(0, stg[0]);
+        __syncthreads();
+    }
+
+    // The two halves meet through LDS (the loop's last barrier has passed).
+    const int64_t ri = (int64_t)it.i * a.nvir_pad, rj = (int64_t)it.j * a.nvir_pad;
+    constexpr int LDG = TILE + 1;
+    double *gx = (double *)&lds[0][0][0][0];
+    int *lx = (int *)&lds[0][0][0][0];
+    if (sq.single) {
+        // exact merge: waves 4-7 hand their int32 level sums to waves 0-3
+        if (prod == 1) {
+#pragma unroll
+            for (int l = 0; l < kLevels; ++l)
+#pragma unroll
+                for (int x = 0; x < NB; ++x)
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                            const int cb = L.wc * TW + y * 16 + L.r;
+                            lx[(l * TILE + ra) * TILE + cb] = acc[l][x][y][v];
+                        }
+        }
+        __syncthreads();
+        if (prod == 0) {
+#pragma unroll
+            for (int l = 0; l < kLevels; ++l)
+#pragma unroll
+                for (int x = 0; x < NB; ++x)
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                            const int cb = L.wc * TW + y * 16 + L.r;
+                            acc[l][x][y][v] += lx[(l * TILE + ra) * TILE + cb];
+                        }
+        }
+        __syncthreads();
+    }
+    // Level sums -> FP64 with the row scales of this wave's operand rows:
+    // C1 pairs (i, a) with (j, b), C2 pairs (j, a) with (i, b); a single-product
+    // item's merged C1 is in waves 0-3.
+    const int64_t rA = prod == 0 ? ri : rj, rB = prod == 0 ? rj : ri;
+    v4d g1[NB][NB], g2[NB][NB];
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            const int bb = B0 + L.wc * TW + y * 16 + L.r;
+            const int eb = cv[y] ? a.rexp[rB + bb] : 0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = A0 + L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                const int ea = rv[x] ? a.rexp[rA + aa] : 0;
+                g1[x][y][v] = combine_levels(acc[0][x][y][v], acc[1][x][y][v], acc[2][x][y][v],
+                                             acc[3][x][y][v], acc[4][x][y][v], ea + eb);
+                g2[x][y][v] = 0.0;
+            }
+        }
+    // C2 into waves 0-3: from waves 4-7 (two-product item), or C1 itself
+    // (i == j), or C1 transposed (diagonal tile pair).
+    const bool copy = sq.single && it.i == it.j;
+    if (!copy) {
+        if (prod == (sq.single ? 0 : 1)) {
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                        const int cb = L.wc * TW + y * 16 + L.r;
+                        gx[ra * LDG + cb] = g1[x][y][v];
+                    }
+        }
+        __syncthreads();
+        if (prod == 0) {
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                        const int cb = L.wc * TW + y * 16 + L.r;
+                        g2[x][y][v] = sq.single ? gx[cb * LDG + ra] : gx[ra * LDG + cb];
+                    }
+        }
+    } else if (prod == 0) {
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) g2[x][y] = g1[x][y];
+    }
+    // Waves 4-7 take part in the epilogue's block reduction with nothing to add.
+    bool rve[NB], cve[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        rve[k] = prod == 0 && rv[k];
+        cve[k] = prod == 0 && cv[k];
+    }
+    pair::energy_epilogue<TILE>(a, it, L, g1, g2, rve, cve, pair::RowOf<float>{},
+                                pair::NoCorr{});
+}
+
 }  // namespace
 
 hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st) {
     if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad) return hipErrorInvalidValue;
     if (a.tile == 64)
-        hipLaunchKernelGGL(energy_i8x4_kernel<64>, dim3((unsigned)a.nitems), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(energy_i8x4_wide_kernel, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
     else if (a.tile == 32)
         hipLaunchKernelGGL(energy_i8x4_kernel<32>, dim3((unsigned)a.nitems), dim3(256), 0, st, a);
     else

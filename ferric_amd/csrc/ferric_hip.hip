@@ -237,6 +237,13 @@ size_t packed_elem_bytes(Prec p) {
     return 8;
 }
 
+// Bytes of one packed operand array. The INT8 kernel reads whole TILE-row
+// blocks without a row test, so its buffer carries TILE spare rows.
+size_t packed_op_bytes(const EnergyPlan &pl, size_t nocc, Prec prec) {
+    const size_t rows = nocc * (size_t)pl.nvir_pad + (prec == Prec::I8X4 ? (size_t)pl.tile : 0);
+    return rows * (size_t)pl.naux_pad * packed_elem_bytes(prec);
+}
+
 // Reduced-precision range check (the reference's check_f32_arithmetic_range,
 // rimp2_gpu.rs:280-293, with the whole naux-deep FP32 accumulation as the
 // panel): refuse when max|B| > sqrt(FLT_MAX / naux).
@@ -284,7 +291,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     const size_t nov = nocc * nvir;
     const size_t rows = nocc * (size_t)pl.nvir_pad;
     const size_t eb = packed_elem_bytes(prec);
-    const size_t op_bytes = rows * (size_t)pl.naux_pad * eb;
+    const size_t op_bytes = packed_op_bytes(pl, nocc, prec);
     const size_t nqb = (size_t)pl.naux_pad / 64;
     std::vector<std::pair<int, size_t>> req = {
         {kSlotOp0, op_bytes},
@@ -642,7 +649,7 @@ fh_status fh_rimp2_energy(const double *b_ov, size_t naux, size_t nocc, size_t n
     const size_t bbytes = sizeof(double) * naux * nocc * nvir;
     EnergyPlan pl = plan_energy(nocc, nvir, naux, prec);
     const size_t eb = packed_elem_bytes(prec);
-    const size_t op_bytes = nocc * (size_t)pl.nvir_pad * (size_t)pl.naux_pad * eb;
+    const size_t op_bytes = packed_op_bytes(pl, nocc, prec);
     std::vector<std::pair<int, size_t>> req = {{kSlotStage, bbytes}, {kSlotOp0, op_bytes}};
     if (prec == Prec::BF16X3) req.push_back({kSlotOp1, op_bytes});
     st = reserve(s, req);
