@@ -75,6 +75,7 @@ ALL_PROBES = [
 # rocprofv3 summaries holding the measured HBM bytes per launch of a headline kernel
 TRAFFIC_PROFILE = {
     ("FP64", 64): os.path.join(ROOT, "profiles", "pair_energy_fp64_t64_mixed_100_900_3600_traffic.md"),
+    ("INT8X4", 64): os.path.join(ROOT, "profiles", "pair_energy_int8x4_t64_mixed_100_900_3600_traffic.md"),
 }
 
 

@@ -5,8 +5,10 @@
 // one exponent e_r with max_Q |B^Q_ia| <= (1 - 2^-7) 2^(e_r + 1), and every
 // element is the integer Y = RN(B 2^(30 - e_r)), |Y| <= 2^31 - 2^24, written as
 // four signed byte digits Y = d1 2^24 + d2 2^16 + d3 2^8 + d4 (d2, d3, d4 in
-// [-128, 127], |d1| <= 127). A row's panel of 64 Q values is 256 bytes: its
-// 64 d1, then the 64 d2, the 64 d3 and the 64 d4.
+// [-128, 127], |d1| <= 127). The digits are stored in MFMA fragment order:
+// for each group of 16 packed rows and each 64-deep Q panel one 4 KB block,
+// digit by digit, and within a digit's kilobyte the 16 bytes of lane
+// l = 16 g + r (row r of the group, Q = 64 P + 16 g .. 16 g + 15) at l * 16.
 //
 // Contraction. sum_Q Y_ia Y_jb = sum_L 2^(8 (8 - L)) S_L with the level sums
 // S_L = sum_Q sum_{p + q = L} d_p d'_q. v_mfma_i32_16x16x64_i8 accumulates
@@ -37,6 +39,49 @@ namespace fh {
 namespace {
 
 constexpr int kLevels = 5;  // digit-product levels L = 2..6, index L - 2
+constexpr int kBand = 4;    // tile rows per band of the tile-pair order
+
+// Workgroup b runs on XCD b % 8. Item of workgroup b: with the occupied pairs
+// in groups of eight, XCD x takes pair x of every group (pairs x, x + 8, ...)
+// and walks each pair's tile pairs in order, so at any moment the eight XCDs
+// work on eight neighbouring pairs, which mostly share their i rows: the
+// blocks one XCD misses in its L2 are often in the shared MALL from another.
+// Pairs past the last whole group keep the plain order. Bijective on
+// [0, nitems).
+__device__ __forceinline__ int64_t pair_interleave(int64_t b, int64_t ntp, int64_t npairs) {
+    const int64_t head = npairs / kXcds * kXcds * ntp;
+    if (b >= head) return b;
+    const int64_t x = b % kXcds, l = b / kXcds;
+    return ((l / ntp) * kXcds + x) * ntp + l % ntp;
+}
+
+// Tile pair t over nt tiles (tA <= tB), this kernel's order: the off-diagonal
+// pairs in bands of kBand consecutive tA rows, tB ascending within a band and
+// tA ascending within each tB; then the nt diagonal pairs. Consecutive items
+// share the B-side blocks of one tB and revisit the band's few A-side blocks
+// every kBand items, so both stay in L2 between their uses (row-major order
+// revisits a B-side block only one whole tA row later). Bijective on
+// [0, nt (nt + 1) / 2).
+__device__ __forceinline__ void decode_tile_pair_banded(int t, int nt, int &ta, int &tb) {
+    const int noff = nt * (nt - 1) / 2;
+    ta = tb = 0;
+    if (t >= noff) {
+        ta = tb = t - noff;
+        return;
+    }
+    for (int r0 = 0; r0 < nt; r0 += kBand) {
+        const int r1 = min(r0 + kBand, nt);
+        for (int b = r0 + 1; b < nt; ++b) {
+            const int n = min(b, r1) - r0;  // rows of the band above the diagonal in column b
+            if (t < n) {
+                ta = r0 + t;
+                tb = b;
+                return;
+            }
+            t -= n;
+        }
+    }
+}
 
 // sum_L 2^(8 (8 - L)) S_L * 2^(e - 60), summed from the smallest level up.
 __device__ __forceinline__ double combine_levels(int s2, int s3, int s4, int s5, int s6, int e) {
@@ -49,272 +94,77 @@ __device__ __forceinline__ double combine_levels(int s2, int s3, int s4, int s5,
     return ldexp(t, e - 12);
 }
 
+// One work item on 8 waves (512 threads): waves 0-3 accumulate C1 (operand
+// rows (i, A) and (j, B)), waves 4-7 C2 ((j, A) and (i, B)), each wave one
+// TILE/2-square quadrant; for a single-product item waves 4-7 take the odd
+// panels of C1 instead. Every fragment (one digit of 16 rows x 64 Q) is one
+// coalesced 1 KB wave load straight into registers, so the main loop has no
+// LDS traffic and no barrier: each wave streams its own fragments one panel
+// ahead (double-buffered registers) while two waves share each SIMD. Blocks
+// of a last tile narrower than TILE read the next rows or the buffer's TILE
+// spare rows; they feed only the 16-row blocks that rv / cv switch off. The
+// halves meet through LDS after the loop (an exact int32 merge of the level
+// sums for a single-product item; FP64 C2 for a two-product item), and waves
+// 0-3 run the energy epilogue from LDS-staged orbital energies and row
+// exponents.
 template <int TILE>
-__global__ __launch_bounds__(256) void energy_i8x4_kernel(EnergyArgs a) {
-    constexpr int KP = 64;             // Q values per panel (one MFMA K)
-    constexpr int TW = TILE / 2;
-    constexpr int NB = TW / 16;
-    constexpr int CPR = 16;            // 16-byte chunks per row and panel (4 digits x 64)
-    constexpr int OPCH = TILE * CPR;
-    constexpr int NCH = 4 * OPCH;
-    constexpr int CPT = (NCH + 255) / 256;
+__global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
+    constexpr int TW = TILE / 2, NB = TW / 16;
+    constexpr int FB = 4096;  // one (16-row group, 64-deep panel) block: 4 digits x 1 KB
 
-    // Chunk c of row r is stored at slot c ^ (r & 15), so that the fragment
-    // reads of 16 consecutive rows hit 16 distinct slots.
-    __shared__ __attribute__((aligned(16))) v4i lds[2][4][TILE][CPR];
+    __shared__ __attribute__((aligned(16))) int xbuf[kLevels * TILE * TILE];
+    // Epilogue inputs, fetched at the start: the virtual energies of tiles A
+    // and B and the row exponents of the four operand row sets.
+    __shared__ double eps_t[2][TILE];
+    __shared__ int rexp_t[4][TILE];
 
     pair::Item it;
-    it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
+    it.id = pair_interleave((int64_t)blockIdx.x, a.ntp, a.nitems / a.ntp);
     decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
-    decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
+    decode_tile_pair_banded((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
 
     const int8_t *__restrict__ bp = (const int8_t *)a.op0;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
-    const pair::SplitQ sq(it, a.naux_pad, KP);
-    // Operand rows as in pair_body.h: 0 = (i, A), 1 = (j, B), 2 = (j, A),
-    // 3 = (i, B), or 0 and 1 one panel ahead for a single-product item.
-    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
-    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
-    const int64_t ldr = (int64_t)a.naux_pad * 4;  // bytes per packed row
-    const int tid = threadIdx.x;
+    const pair::SplitQ sq(it, a.naux_pad, 64);
+    const int np = a.naux_pad / 64, ng = a.nvir_pad / 16;
+    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+    const int prod = wid >> 2, quad = wid & 3;
+    const pair::Lane L{A0, B0, quad >> 1, quad & 1, lane % 16, lane / 16};
 
-    // Global -> LDS staging. Thread t owns chunk ch = t % 16 of rows
-    // r0 + 16 k (r0 = t / 16) of every operand, so its four row pointers are
-    // loop-invariant and every load is pointer + uniform offset. Rows past
-    // nvir_pad (a last tile narrower than TILE) are read from the next occupied
-    // index or from the buffer's TILE spare rows: they feed only the 16-row
-    // blocks that rv / cv switch off, so no row test is needed.
-    constexpr int RPS = 256 / CPR;  // rows per pass of the workgroup (16)
-    constexpr int SPO = TILE / RPS; // passes per operand
-    static_assert(CPT == 4 * SPO && NCH == 256 * CPT, "one chunk per thread and pass");
-    const int ch = tid % CPR, r0 = tid / CPR;
-    const int8_t *src[4];
-    src[0] = bp + (base0 + r0) * ldr + ch * 16;
-    src[1] = bp + (base1 + r0) * ldr + ch * 16;
-    src[2] = bp + (base2 + r0) * ldr + ch * 16;
-    src[3] = bp + (base3 + r0) * ldr + ch * 16;
-    v4i stg[CPT];
-    auto gload = [&](int q0) {
-        const int q23 = q0 + sq.off23;
-        // uniform: a single-product item's extra panel past the end is zeros
-        const bool hi_ok = q23 < a.naux_pad;
+    // Operand rows of this wave: C1 (i, A) x (j, B), C2 (j, A) x (i, B); a
+    // single-product item's waves 4-7 read C1's rows at the odd panels.
+    const bool c2 = prod == 1 && !sq.single;
+    const int occA = c2 ? it.j : it.i, occB = c2 ? it.i : it.j;
+    const int pstep = sq.single ? 2 : 1, poff = (prod == 1 && sq.single) ? 1 : 0;
+    const int8_t *pa[NB], *pb[NB];
 #pragma unroll
-        for (int op = 0; op < 4; ++op)
-#pragma unroll
-            for (int k = 0; k < SPO; ++k) {
-                const int64_t off = (int64_t)k * RPS * ldr + (int64_t)(op < 2 ? q0 : q23) * 4;
-                v4i v = {0, 0, 0, 0};
-                if (op < 2 || hi_ok) v = *(const v4i *)(src[op] + off);
-                stg[op * SPO + k] = v;
-            }
-    };
-    // (r0 + 16 k) & 15 == r0: one swizzled slot per thread.
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int op = 0; op < 4; ++op)
-#pragma unroll
-            for (int k = 0; k < SPO; ++k) lds[buf][op][r0 + RPS * k][ch ^ r0] = stg[op * SPO + k];
-    };
+    for (int k = 0; k < NB; ++k) {
+        pa[k] = bp + (int64_t)(occA * ng + (A0 + L.wr * TW) / 16 + k) * np * FB + lane * 16;
+        pb[k] = bp + (int64_t)(occB * ng + (B0 + L.wc * TW) / 16 + k) * np * FB + lane * 16;
+    }
 
-    const int lane = tid % kWave, wid = tid / kWave;
-    const pair::Lane L{A0, B0, wid / 2, wid % 2, lane % 16, lane / 16};
+    if (tid < 2 * TILE) {
+        const int c = tid % TILE, v = (tid < TILE ? A0 : B0) + c;
+        eps_t[tid / TILE][c] = v < a.nvir ? a.eps_vir[v] : 0.0;
+    } else if (tid < 6 * TILE) {
+        const int t = tid - 2 * TILE, op = t / TILE, c = t % TILE;
+        // operand row sets 0 (i, A), 1 (j, B), 2 (j, A), 3 (i, B); a
+        // single-product item's 2 and 3 are 0 and 1
+        const bool swap = op >= 2 && !sq.single;
+        const int occ = ((op & 1) ^ swap) ? it.j : it.i;
+        const int v = ((op & 1) ? B0 : A0) + c;
+        rexp_t[op][c] = v < a.nvir_pad ? a.rexp[(int64_t)occ * a.nvir_pad + v] : 0;
+    }
+
     bool rv[NB], cv[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
         cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
     }
-    v4i acc1[kLevels][NB][NB], acc2[kLevels][NB][NB];
-#pragma unroll
-    for (int l = 0; l < kLevels; ++l)
-#pragma unroll
-        for (int x = 0; x < NB; ++x)
-#pragma unroll
-            for (int y = 0; y < NB; ++y) {
-                acc1[l][x][y] = (v4i){0, 0, 0, 0};
-                acc2[l][x][y] = (v4i){0, 0, 0, 0};
-            }
-
-    // Lane (r, g) supplies the 16 bytes [16 g, 16 g + 16) of digit d's 64 for
-    // A rows and B columns alike, so every MFMA contracts matching Q.
-    auto frag = [&](int buf, int op, int row, int d) -> v4i {
-        return lds[buf][op][row][(d * 4 + L.g) ^ (row & 15)];
-    };
-    // FULL: every 16x16 block of the wave lies inside nvir_pad (the common
-    // case), so the MFMA sequence needs no per-block test.
-    auto compute = [&](int buf, auto full) {
-        v4i f0[4][NB], f2[4][NB], f1[4][NB], f3[4][NB];
-#pragma unroll
-        for (int d = 0; d < 4; ++d)
-#pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                const int ra = L.wr * TW + k * 16 + L.r, rb = L.wc * TW + k * 16 + L.r;
-                f0[d][k] = frag(buf, 0, ra, d);
-                f2[d][k] = frag(buf, 2, ra, d);
-                f1[d][k] = frag(buf, 1, rb, d);
-                f3[d][k] = frag(buf, 3, rb, d);
-            }
-#pragma unroll
-        for (int x = 0; x < NB; ++x)
-#pragma unroll
-            for (int y = 0; y < NB; ++y)
-                if (decltype(full)::value || (rv[x] && cv[y])) {
-#pragma unroll
-                    for (int p = 0; p < 4; ++p)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (p + q < kLevels) {
-                                acc1[p + q][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                    f0[p][x], f1[q][y], acc1[p + q][x][y], 0, 0, 0);
-                                acc2[p + q][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                    f2[p][x], f3[q][y], acc2[p + q][x][y], 0, 0, 0);
-                            }
-                }
-    };
-
     bool full = true;
 #pragma unroll
     for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
-
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int p = 0; p < sq.iters; ++p) {
-        const int cur = p & 1;
-        if (p + 1 < sq.iters) gload((p + 1) * sq.step);
-        if (full)
-            compute(cur, std::true_type{});
-        else
-            compute(cur, std::false_type{});
-        if (p + 1 < sq.iters) lstore(cur ^ 1);
-        __syncthreads();
-    }
-
-    // A single-product item's two accumulator sets both hold C1 (even and odd
-    // panels): merge them exactly in int32 before the conversion.
-    if (sq.single) {
-#pragma unroll
-        for (int l = 0; l < kLevels; ++l)
-#pragma unroll
-            for (int x = 0; x < NB; ++x)
-#pragma unroll
-                for (int y = 0; y < NB; ++y) {
-                    acc1[l][x][y] += acc2[l][x][y];
-                    acc2[l][x][y] = (v4i){0, 0, 0, 0};
-                }
-    }
-
-    // Level sums -> FP64 g with the row scales: C1(a, b) pairs rows (i, a)
-    // and (j, b), C2(a, b) rows (j, a) and (i, b).
-    const int64_t ri = (int64_t)it.i * a.nvir_pad, rj = (int64_t)it.j * a.nvir_pad;
-    v4d g1[NB][NB], g2[NB][NB];
-#pragma unroll
-    for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
-            const int ej_b = cv[y] ? a.rexp[rj + bb] : 0;
-            const int ei_b = cv[y] ? a.rexp[ri + bb] : 0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int aa = L.A0 + L.wr * TW + x * 16 + out_row<float>(L.g, v);
-                const int ei_a = rv[x] ? a.rexp[ri + aa] : 0;
-                const int ej_a = rv[x] ? a.rexp[rj + aa] : 0;
-                g1[x][y][v] = combine_levels(acc1[0][x][y][v], acc1[1][x][y][v], acc1[2][x][y][v],
-                                             acc1[3][x][y][v], acc1[4][x][y][v],
-                                             ei_a + ej_b);
-                g2[x][y][v] = sq.single
-                                  ? 0.0
-                                  : combine_levels(acc2[0][x][y][v], acc2[1][x][y][v],
-                                                   acc2[2][x][y][v], acc2[3][x][y][v],
-                                                   acc2[4][x][y][v], ej_a + ei_b);
-            }
-        }
-    if (sq.single) {
-        static_assert(sizeof(lds) >= sizeof(double) * TW * (TILE + 1), "LDS holds a C1 half-tile");
-        pair::derive_c2<TILE, double>(it, L, g1, g2, pair::RowOf<float>{},
-                                      (double *)&lds[0][0][0][0]);
-    }
-    pair::energy_epilogue<TILE>(a, it, L, g1, g2, rv, cv, pair::RowOf<float>{}, pair::NoCorr{});
-}
-
-// 64-wide tile on 8 waves: waves 0-3 accumulate C1 (operands 0 and 1), waves
-// 4-7 accumulate C2 (operands 2 and 3), each wave one 32x32 quadrant. Half the
-// accumulators per wave of the 4-wave kernel let two waves share each SIMD, so
-// one wave's LDS fragment reads overlap the other's MFMAs, and the registers
-// left over hold a second staged panel: global loads run two panels ahead of
-// the MFMAs. For a single-product item waves 4-7 take the odd panels of C1.
-// The halves meet through LDS after the loop (exact int32 merge of the level
-// sums for a single-product item; FP64 C2 for a two-product item), and waves
-// 0-3 run the energy epilogue.
-__global__ __launch_bounds__(512) void energy_i8x4_wide_kernel(EnergyArgs a) {
-    constexpr int TILE = 64, KP = 64;
-    constexpr int TW = 32, NB = 2;
-    constexpr int CPR = 16;             // 16-byte chunks per row and panel
-    constexpr int RPS = 512 / CPR;      // rows per pass of the workgroup (32)
-    constexpr int SPO = TILE / RPS;     // passes per operand (2)
-    constexpr int CPT = 4 * SPO;        // staged chunks per thread and panel (8)
-
-    __shared__ __attribute__((aligned(16))) v4i lds[2][4][TILE][CPR];
-
-    pair::Item it;
-    it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
-    decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
-    decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
-
-    const int8_t *__restrict__ bp = (const int8_t *)a.op0;
-    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
-    const pair::SplitQ sq(it, a.naux_pad, KP);
-    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
-    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
-    const int64_t ldr = (int64_t)a.naux_pad * 4;
-    const int tid = threadIdx.x;
-
-    // Staging as in the 4-wave kernel (rows past nvir_pad only feed switched-off
-    // blocks; the buffer has TILE spare rows): thread t owns chunk t % 16 of
-    // rows t / 16 + 32 k of every operand.
-    const int ch = tid % CPR, r0 = tid / CPR;
-    const int8_t *src[4];
-    src[0] = bp + (base0 + r0) * ldr + ch * 16;
-    src[1] = bp + (base1 + r0) * ldr + ch * 16;
-    src[2] = bp + (base2 + r0) * ldr + ch * 16;
-    src[3] = bp + (base3 + r0) * ldr + ch * 16;
-    v4i stg[2][CPT];
-    auto gload = [&](int p, v4i(&s)[CPT]) {
-        const int q0 = p * sq.step, q23 = q0 + sq.off23;
-        const bool hi_ok = q23 < a.naux_pad;  // uniform
-#pragma unroll
-        for (int op = 0; op < 4; ++op)
-#pragma unroll
-            for (int k = 0; k < SPO; ++k) {
-                const int64_t off = (int64_t)k * RPS * ldr + (int64_t)(op < 2 ? q0 : q23) * 4;
-                v4i v = {0, 0, 0, 0};
-                if (op < 2 || hi_ok) v = *(const v4i *)(src[op] + off);
-                s[op * SPO + k] = v;
-            }
-    };
-    auto lstore = [&](int buf, const v4i(&s)[CPT]) {
-#pragma unroll
-        for (int op = 0; op < 4; ++op)
-#pragma unroll
-            for (int k = 0; k < SPO; ++k)
-                lds[buf][op][r0 + RPS * k][ch ^ (r0 & 15)] = s[op * SPO + k];
-    };
-
-    const int lane = tid % kWave, wid = tid / kWave;
-    const int prod = wid >> 2, quad = wid & 3;
-    const pair::Lane L{A0, B0, quad >> 1, quad & 1, lane % 16, lane / 16};
-    const int opA = 2 * prod, opB = 2 * prod + 1;
-    bool rv[NB], cv[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
-        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
-    }
     v4i acc[kLevels][NB][NB];
 #pragma unroll
     for (int l = 0; l < kLevels; ++l)
@@ -323,65 +173,56 @@ __global__ __launch_bounds__(512) void energy_i8x4_wide_kernel(EnergyArgs a) {
 #pragma unroll
             for (int y = 0; y < NB; ++y) acc[l][x][y] = (v4i){0, 0, 0, 0};
 
-    auto compute = [&](int buf, auto full) {
-        v4i fa[4][NB], fb[4][NB];
+    v4i fa[2][4][NB], fb[2][4][NB];
+    auto fetch = [&](int P, v4i(&ra)[4][NB], v4i(&rb)[4][NB]) {
+        const int64_t off = (int64_t)P * FB;
 #pragma unroll
         for (int d = 0; d < 4; ++d)
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const int ra = L.wr * TW + k * 16 + L.r, rb = L.wc * TW + k * 16 + L.r;
-                fa[d][k] = lds[buf][opA][ra][(d * 4 + L.g) ^ L.r];
-                fb[d][k] = lds[buf][opB][rb][(d * 4 + L.g) ^ L.r];
+                ra[d][k] = *(const v4i *)(pa[k] + off + d * 1024);
+                rb[d][k] = *(const v4i *)(pb[k] + off + d * 1024);
             }
+    };
+    // FULL: every 16x16 block of the wave lies inside nvir_pad (the common
+    // case), so the MFMA sequence needs no per-block test.
+    auto mma = [&](const v4i(&ra)[4][NB], const v4i(&rb)[4][NB], auto fullc) {
 #pragma unroll
         for (int x = 0; x < NB; ++x)
 #pragma unroll
             for (int y = 0; y < NB; ++y)
-                if (decltype(full)::value || (rv[x] && cv[y])) {
+                if (decltype(fullc)::value || (rv[x] && cv[y])) {
 #pragma unroll
                     for (int p = 0; p < 4; ++p)
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
                             if (p + q < kLevels)
                                 acc[p + q][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                    fa[p][x], fb[q][y], acc[p + q][x][y], 0, 0, 0);
+                                    ra[p][x], rb[q][y], acc[p + q][x][y], 0, 0, 0);
                 }
     };
-    bool full = true;
-#pragma unroll
-    for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
-    auto panel = [&](int buf) {
+    auto compute = [&](const v4i(&ra)[4][NB], const v4i(&rb)[4][NB]) {
         if (full)
-            compute(buf, std::true_type{});
+            mma(ra, rb, std::true_type{});
         else
-            compute(buf, std::false_type{});
+            mma(ra, rb, std::false_type{});
     };
-
-    // Two panels in flight: at iteration p, LDS buffer p & 1 holds panel p,
-    // stg[(p + 1) & 1] panel p + 1 (landing), and panel p + 2 is fetched into
-    // stg[p & 1]. The loop is unrolled by two so the staging index is static.
+    // Iteration p covers panel p * pstep + poff (a single-product item's odd
+    // panel past the end is skipped by its waves); unrolled by two so the
+    // register buffer of each step is static.
     const int iters = sq.iters;
-    gload(0, stg[0]);
-    if (iters > 1) gload(1, stg[1]);
-    lstore(0, stg[0]);
-    __syncthreads();
+    auto valid = [&](int p) { return p < iters && p * pstep + poff < np; };
+    if (valid(0)) fetch(poff, fa[0], fb[0]);
     for (int p = 0; p < iters; p += 2) {
-        if (p + 2 < iters) gload(p + 2, stg[0]);
-        panel(0);
-        if (p + 1 < iters) lstore(1, stg[1]);
-        __syncthreads();
-        if (p + 1 >= iters) break;
-        if (p + 3 < iters) gload(p + 3, stg[1]);
-        panel(1);
-        if (p + 2 < iters) lstore(0, stg[0]);
-        __syncthreads();
+        if (valid(p + 1)) fetch((p + 1) * pstep + poff, fa[1], fb[1]);
+        if (valid(p)) compute(fa[0], fb[0]);
+        if (valid(p + 2)) fetch((p + 2) * pstep + poff, fa[0], fb[0]);
+        if (valid(p + 1)) compute(fa[1], fb[1]);
     }
+    __syncthreads();  // eps_t / rexp_t written; every wave done with its loop
 
-    // The two halves meet through LDS (the loop's last barrier has passed).
-    const int64_t ri = (int64_t)it.i * a.nvir_pad, rj = (int64_t)it.j * a.nvir_pad;
     constexpr int LDG = TILE + 1;
-    double *gx = (double *)&lds[0][0][0][0];
-    int *lx = (int *)&lds[0][0][0][0];
+    double *gx = (double *)xbuf;
     if (sq.single) {
         // exact merge: waves 4-7 hand their int32 level sums to waves 0-3
         if (prod == 1) {
@@ -395,7 +236,7 @@ __global__ __launch_bounds__(512) void energy_i8x4_wide_kernel(EnergyArgs a) {
                         for (int v = 0; v < 4; ++v) {
                             const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
                             const int cb = L.wc * TW + y * 16 + L.r;
-                            lx[(l * TILE + ra) * TILE + cb] = acc[l][x][y][v];
+                            xbuf[(l * TILE + ra) * TILE + cb] = acc[l][x][y][v];
                         }
         }
         __syncthreads();
@@ -410,32 +251,28 @@ __global__ __launch_bounds__(512) void energy_i8x4_wide_kernel(EnergyArgs a) {
                         for (int v = 0; v < 4; ++v) {
                             const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
                             const int cb = L.wc * TW + y * 16 + L.r;
-                            acc[l][x][y][v] += lx[(l * TILE + ra) * TILE + cb];
+                            acc[l][x][y][v] += xbuf[(l * TILE + ra) * TILE + cb];
                         }
         }
         __syncthreads();
     }
-    // Level sums -> FP64 with the row scales of this wave's operand rows:
-    // C1 pairs (i, a) with (j, b), C2 pairs (j, a) with (i, b); a single-product
-    // item's merged C1 is in waves 0-3.
-    const int64_t rA = prod == 0 ? ri : rj, rB = prod == 0 ? rj : ri;
+    // Level sums -> FP64 with the row scales of this wave's operand rows.
+    const int eA = c2 ? 2 : 0, eB = c2 ? 3 : 1;
     v4d g1[NB][NB], g2[NB][NB];
 #pragma unroll
     for (int x = 0; x < NB; ++x)
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
-            const int bb = B0 + L.wc * TW + y * 16 + L.r;
-            const int eb = cv[y] ? a.rexp[rB + bb] : 0;
+            const int eb = rexp_t[eB][L.wc * TW + y * 16 + L.r];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int aa = A0 + L.wr * TW + x * 16 + out_row<float>(L.g, v);
-                const int ea = rv[x] ? a.rexp[rA + aa] : 0;
+                const int ea = rexp_t[eA][L.wr * TW + x * 16 + out_row<float>(L.g, v)];
                 g1[x][y][v] = combine_levels(acc[0][x][y][v], acc[1][x][y][v], acc[2][x][y][v],
                                              acc[3][x][y][v], acc[4][x][y][v], ea + eb);
                 g2[x][y][v] = 0.0;
             }
         }
-    // C2 into waves 0-3: from waves 4-7 (two-product item), or C1 itself
+    // C2 into waves 0-3: from waves 4-7 (two-product item), C1 itself
     // (i == j), or C1 transposed (diagonal tile pair).
     const bool copy = sq.single && it.i == it.j;
     if (!copy) {
@@ -470,25 +307,47 @@ __global__ __launch_bounds__(512) void energy_i8x4_wide_kernel(EnergyArgs a) {
 #pragma unroll
             for (int y = 0; y < NB; ++y) g2[x][y] = g1[x][y];
     }
-    // Waves 4-7 take part in the epilogue's block reduction with nothing to add.
-    bool rve[NB], cve[NB];
+    // The energy terms (pair::energy_epilogue's, with the virtual energies
+    // from LDS); waves 4-7 join the block reduction with nothing to add.
+    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
+    const bool kap = !isnan(a.kappa);
+    KSum os, ss;
+    if (prod == 0) {
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        rve[k] = prod == 0 && rv[k];
-        cve[k] = prod == 0 && cv[k];
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) {
+                if (!(rv[x] && cv[y])) continue;
+                const int cb = L.wc * TW + y * 16 + L.r;
+                if (B0 + cb >= a.nvir) continue;
+                const double eb = eps_t[1][cb];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                    if (A0 + ra >= a.nvir) continue;
+                    const double d = eij - eps_t[0][ra] - eb;
+                    const double g1v = g1[x][y][v], g2v = g2[x][y][v];
+                    const double inv = 1.0 / d;
+                    const double dd = g1v - g2v;
+                    const double dmp = pair::damping(kap, a.kappa, d);
+                    os.add(dmp * ((g1v * g1v + g2v * g2v) * inv));
+                    ss.add(dmp * ((dd * dd) * inv));
+                }
+            }
     }
-    pair::energy_epilogue<TILE>(a, it, L, g1, g2, rve, cve, pair::RowOf<float>{},
-                                pair::NoCorr{});
+    const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
+    pair::block_store(os, ss, w, a.item_partials + 4 * it.id);
 }
 
 }  // namespace
 
 hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st) {
-    if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad) return hipErrorInvalidValue;
+    if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad || a.nvir_pad % 16 != 0)
+        return hipErrorInvalidValue;
     if (a.tile == 64)
-        hipLaunchKernelGGL(energy_i8x4_wide_kernel, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
+        hipLaunchKernelGGL(energy_i8x4_kernel<64>, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
     else if (a.tile == 32)
-        hipLaunchKernelGGL(energy_i8x4_kernel<32>, dim3((unsigned)a.nitems), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(energy_i8x4_kernel<32>, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -18,8 +18,8 @@
 // OCP FP8 E4M3 with one power-of-two (E8M0) scale per (occupied i, 64-row Q
 // block), and INT8 digit slices: per packed row one exponent f with
 // max_Q |x| <= (1 - 2^-7) 2^(f + 1), and per element the four signed byte
-// digits of Y = RN(x 2^(30 - f)); each 64-deep Q panel of a row is 256 bytes,
-// digit by digit (energy_i8.hip).
+// digits of Y = RN(x 2^(30 - f)), stored in MFMA fragment order: one 4 KB
+// block per (16 packed rows, 64-deep Q panel), digit by digit (energy_i8.hip).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -330,11 +330,15 @@ __global__ __launch_bounds__(256) void pack_i8x4_kernel(const Tin *__restrict__ 
         y = (y - d3) >> 8;
         const int d2 = (int)(int8_t)(y & 0xff);
         const int d1 = (y - d2) >> 8;
-        int8_t *o = out + orow * ((int64_t)naux_pad * 4) + (int64_t)q0 * 4 + tx;
+        // MFMA fragment order: block (row group orow / 16, panel q0 / 64) of
+        // 4 KB, digit by digit; lane 16 g + r (r = orow % 16, g = tx / 16)
+        // holds Q = q0 + 16 g .. q0 + 16 g + 15 at 16 (16 g + r).
+        int8_t *o = out + ((orow / 16) * (int64_t)(naux_pad / 64) + q0 / 64) * 4096 +
+                    ((tx / 16) * 16 + (int)(orow % 16)) * 16 + tx % 16;
         o[0] = (int8_t)d1;
-        o[64] = (int8_t)d2;
-        o[128] = (int8_t)d3;
-        o[192] = (int8_t)d4;
+        o[1024] = (int8_t)d2;
+        o[2048] = (int8_t)d3;
+        o[3072] = (int8_t)d4;
     }
 }
 
