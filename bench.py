@@ -143,6 +143,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-dressing", action="store_true")
     ap.add_argument("--no-small-configs", action="store_true",
                     help="skip the water-shape and decane-scale legs")
     ap.add_argument("--cpu-rows", default="50:100", help="occupied rows of the CPU sample")
@@ -284,6 +285,34 @@ def main():
             log(f"batched 512 domains {prec.name}: {ms:.2f} ms, {row['domains_per_s']:.0f} "
                 f"domains/s, {row['tflops']:.1f} TFLOP/s")
         batched["flops_per_step"] = ds.flops
+
+    # The dressing / MO transform upstream of the pair stage at the headline shape:
+    # (Q|mu nu) with naux = 3600 and nao = nocc + nvir = 1000 (random values),
+    # dressed into B (3600, 100*900) by three FP64 rocBLAS GEMMs.
+    dressing = {}
+    if not args.no_dressing:
+        from ferric_amd.dressing import stream_dressed_mo_band
+
+        nao, nl, nr, nq = p.nocc + p.nvir, p.nocc, p.nvir, p.naux
+        gen = torch.Generator(device=dev).manual_seed(7)
+        j3 = torch.randn((nq, nao, nao), dtype=torch.float64, device=dev, generator=gen)
+        dm = torch.randn((nq, nq), dtype=torch.float64, device=dev, generator=gen) / nq
+        cl = torch.randn((nao, nl), dtype=torch.float64, device=dev, generator=gen)
+        cr = torch.randn((nao, nr), dtype=torch.float64, device=dev, generator=gen)
+        out_b = stream_dressed_mo_band(j3, dm, cl, cr)   # warm (rocBLAS set-up)
+        torch.cuda.synchronize()
+        k = max(3, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            out_b = stream_dressed_mo_band(j3, dm, cl, cr)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / k
+        fl = 2.0 * nq * nao * nao * nl + 2.0 * nq * nao * nl * nr + 2.0 * nq * nq * nl * nr
+        dressing = {"shape": f"naux {nq}, nao {nao}, {nl} x {nr}", "ms_per_step": ms,
+                    "tflops": fl / (ms * 1e-3) / 1e12, "flops_per_step": fl}
+        log(f"dressing {dressing['shape']}: {ms:.1f} ms, {dressing['tflops']:.1f} TFLOP/s FP64")
+        del j3, dm, cl, cr, out_b
+        torch.cuda.empty_cache()
 
     # The other named shapes: water (5/19/84, launch-latency bound), decane scale
     # (41/209/960) in FP64 and FP32, and the decane-scale adjoint in FP64 and BF16x3.
@@ -448,6 +477,7 @@ def main():
         "variants": res,
         "energy_adjoint": adj,
         "batched_domains_512": batched,
+        "dressing": dressing,
         "small_configs": small,
         "roofline": {
             "bound": "mfma",
