@@ -233,12 +233,14 @@ def main():
         log(f"{prec.name}: {ms:.2f} ms/step (kernel {c_ms:.2f} ms, pack {p_ms:.2f}, reduce "
             f"{r_ms:.3f}) {r['tflops']:.1f} TFLOP/s  dE {r['dE_vs_fp64']:+.3e}")
 
-    # Energy + adjoint (FP64 vs BF16x3): the time of E_corr together with dE/dB, dE/deps.
+    # Energy + adjoint (FP64, BF16x3, INT8x4): the time of E_corr together with
+    # dE/dB and dE/deps, gamma compared with FP64's.
     adj = {}
     if not args.no_adjoint:
         from ferric_amd.rimp2 import energy_adjoint
 
-        for prec in (Precision.FP64, Precision.BF16X3):
+        g64 = None
+        for prec in (Precision.FP64, Precision.BF16X3, Precision.INT8X4):
             energy_adjoint(b, p.eps, p.nocc, p.nvir, precision=prec)   # warm
             torch.cuda.synchronize()
             k = max(1, min(args.steps, 3))
@@ -249,11 +251,17 @@ def main():
             ms = (time.perf_counter() - t0) * 1e3 / k
             adj[prec.name] = {"ms_per_step": ms, "E": e_tot, "tflops_3FE": 3 * F / (ms * 1e-3) / 1e12,
                               "gamma_abs_max": float(gam.abs().max())}
-            del gam
+            if g64 is None:
+                g64 = gam
+            else:
+                adj[prec.name]["dE_vs_fp64"] = e_tot - adj["FP64"]["E"]
+                adj[prec.name]["max_rel_dGamma_vs_fp64"] = float((gam - g64).abs().max()
+                                                                 / g64.abs().max())
+                adj[f"speedup_{prec.name.lower()}_vs_fp64"] = (adj["FP64"]["ms_per_step"] / ms)
+                del gam
             log(f"energy+adjoint {prec.name}: {ms:.1f} ms ({adj[prec.name]['tflops_3FE']:.1f} "
-                f"TFLOP/s on 3 F_E)")
-        adj["BF16X3"]["dE_vs_fp64"] = adj["BF16X3"]["E"] - adj["FP64"]["E"]
-        adj["speedup_bf16x3_vs_fp64"] = adj["FP64"]["ms_per_step"] / adj["BF16X3"]["ms_per_step"]
+                f"TFLOP/s on 3 F_E) dE {adj[prec.name].get('dE_vs_fp64', 0.0):+.3e}")
+        del g64
 
     # Batched local-MP2 domains (512, one persistent launch per tile class, hipGraph).
     batched = {}

@@ -16,8 +16,8 @@ from .rimp2 import energy_adjoint, spin_components
 
 
 def _adjoint_precision(p: Precision) -> Precision:
-    # The adjoint runs in FP64 or BF16x3: BF16x3 stays, everything else is FP64.
-    return Precision.BF16X3 if p == Precision.BF16X3 else Precision.FP64
+    # The adjoint runs in FP64, BF16x3 or INT8x4: those stay, everything else is FP64.
+    return p if p in (Precision.BF16X3, Precision.INT8X4) else Precision.FP64
 
 
 class RiMp2Function(torch.autograd.Function):

@@ -580,7 +580,14 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
                               double *gamma_dev, double *deps) {
     const bool in_f32 = b_dtype == FH_DT_F32;
     const bool f64 = prec == Prec::F64;
+    // INT8X4: phase A on the INT8 digit-slice kernel (the energy and T~ with
+    // FP64-class accuracy), phases B/C on the BF16x3 GEMMs (T~ and B kept FP32).
+    const bool i8 = prec == Prec::I8X4;
+    const Prec gprec = f64 ? Prec::F64 : Prec::BF16X3;
     EnergyPlan pl = plan_energy(nocc, nvir, naux, prec);
+    if (i8 && pl.naux_pad > kI8MaxNauxPad)
+        return fail(FH_RANGE, "naux_pad %d beyond the INT8 slices' int32 range (%d)", pl.naux_pad,
+                    kI8MaxNauxPad);
     // The FP64 adjoint epilogue does not fit a 128-wide tile in the register
     // file (it would spill); 64-wide tiles run two workgroups per CU.
     if (f64 && pl.tile > 64) pl.tile = 64;
@@ -589,11 +596,10 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     pl.nvir_pad = (int)((nvir + 31) / 32 * 32);
     pl.ntile = (pl.nvir_pad + pl.tile - 1) / pl.tile;
     pl.ntp = pl.ntile * (pl.ntile + 1) / 2;
-    const int gtile = gemm_tile(prec);
+    const int gtile = gemm_tile(gprec);
     const int m_pad = (int)((naux + gtile - 1) / gtile * gtile);
     const size_t rows = nocc * (size_t)pl.nvir_pad;
-    const size_t eb = packed_elem_bytes(prec);
-    const size_t op_bytes = rows * (size_t)pl.naux_pad * eb;
+    const size_t op_bytes = packed_op_bytes(pl, nocc, prec);
     const size_t tb = f64 ? 8 : 4;
     const size_t b2_bytes = (size_t)m_pad * rows * tb;
     const int64_t ld_t = (int64_t)rows;  // nocc * nvir_pad
@@ -627,10 +633,11 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
         {kSlotSplit, sizeof(double) * (size_t)splits * m_pad * pl.nvir_pad},
         {kSlotRed, sizeof(double) * (2 * (size_t)nchunks_max * nvir + 6 * nocc)},
     };
-    if (!f64) {
+    if (prec == Prec::BF16X3) {
         req.push_back({kSlotOp1, op_bytes});
         req.push_back({kSlotDiag, sizeof(double) * rows});
     }
+    if (i8) req.push_back({kSlotScale, sizeof(int) * rows});
     fh_status st = reserve(s, req);
     if (st != FH_OK) return st;
     if (!f64) {
@@ -656,11 +663,12 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     pa.nvir = (int)nvir;
     pa.nvir_pad = pl.nvir_pad;
     pa.naux_pad = pl.naux_pad;
-    pa.fmt = f64 ? PackFmt::F64 : PackFmt::BF16X2;
+    pa.fmt = f64 ? PackFmt::F64 : i8 ? PackFmt::I8X4 : PackFmt::BF16X2;
     pa.out0 = s.slots[kSlotOp0].ptr;
-    pa.out1 = f64 ? nullptr : s.slots[kSlotOp1].ptr;
+    pa.out1 = prec == Prec::BF16X3 ? s.slots[kSlotOp1].ptr : nullptr;
     pa.scale = nullptr;
-    pa.lo_sq = f64 ? nullptr : (double *)s.slots[kSlotDiag].ptr;
+    pa.lo_sq = prec == Prec::BF16X3 ? (double *)s.slots[kSlotDiag].ptr : nullptr;
+    pa.rexp = i8 ? (int *)s.slots[kSlotScale].ptr : nullptr;
     hipError_t he = launch_pack(pa, stream);
     if (he != hipSuccess) return fail(FH_KERNEL, "pack kernel: %s", hipGetErrorString(he));
     {
@@ -701,6 +709,7 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     ea.op1 = pa.out1;
     ea.scale = nullptr;
     ea.diag_corr = pa.lo_sq;
+    ea.rexp = pa.rexp;
     ea.nocc = (int)nocc;
     ea.nvir = (int)nvir;
     ea.nvir_pad = pl.nvir_pad;
@@ -724,7 +733,7 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     ea.vec_stride = vec_stride;
 
     GemmArgs ga;
-    ga.prec = prec;
+    ga.prec = gprec;
     ga.b2 = s.slots[kSlotStage2].ptr;
     ga.ld_b2 = (int64_t)rows;
     ga.tbuf = ea.tbuf;
@@ -822,8 +831,8 @@ static fh_status adjoint_checks(size_t naux, size_t nocc, size_t nvir, size_t n_
                                 size_t first_occ, size_t nocc_total, double c_os, double c_ss,
                                 int precision, Prec *prec, bool *empty) {
     if (!to_prec(precision, prec)) return fail(FH_LAYOUT, "unknown precision %d", precision);
-    if (*prec != Prec::F64 && *prec != Prec::BF16X3)
-        return fail(FH_UNSUPPORTED, "the adjoint runs in FP64 or BF16x3 only");
+    if (*prec != Prec::F64 && *prec != Prec::BF16X3 && *prec != Prec::I8X4)
+        return fail(FH_UNSUPPORTED, "the adjoint runs in FP64, BF16x3 or INT8x4 only");
     if (!std::isfinite(c_os) || !std::isfinite(c_ss))
         return fail(FH_LAYOUT, "spin scaling must be finite, got c_os = %g, c_ss = %g", c_os, c_ss);
     return check_shapes(naux, nocc, nvir, n_eps, first_occ, nocc_total, empty);

@@ -193,6 +193,9 @@ hipError_t launch_adjoint_pairs(const EnergyArgs &a, hipStream_t st) {
         else if (a.tile == 64) e = launch_items(pair_bf16x3_kernel<64, true>, a, st);
         else if (a.tile == 32) e = launch_items(pair_bf16x3_kernel<32, true>, a, st);
         break;
+    case Prec::I8X4:
+        e = launch_adjoint_i8x4(a, st);
+        break;
     default:
         break;
     }

@@ -106,8 +106,10 @@ __device__ __forceinline__ double combine_levels(int s2, int s3, int s4, int s5,
 // halves meet through LDS after the loop (an exact int32 merge of the level
 // sums for a single-product item; FP64 C2 for a two-product item), and waves
 // 0-3 run the energy epilogue from LDS-staged orbital energies and row
-// exponents.
-template <int TILE>
+// exponents. ADJ: phase A of the adjoint for occupied row a.i_fixed (items
+// (i_fixed, j >= i_fixed) x tile pairs), ending in the T~ epilogue of
+// pair_body.h instead of the energy terms.
+template <int TILE, bool ADJ>
 __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
     constexpr int TW = TILE / 2, NB = TW / 16;
     constexpr int FB = 4096;  // one (16-row group, 64-deep panel) block: 4 digits x 1 KB
@@ -120,8 +122,16 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
 
     pair::Item it;
     it.id = pair_interleave((int64_t)blockIdx.x, a.ntp, a.nitems / a.ntp);
-    decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
-    decode_tile_pair_banded((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
+    if (ADJ) {
+        // the adjoint's row reduction reads each item's row / column sums at
+        // item id (j - i) * ntp + t with t in decode_tile_pair's order
+        it.i = a.i_fixed;
+        it.j = a.i_fixed + (int)(it.id / a.ntp);
+        decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
+    } else {
+        decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
+        decode_tile_pair_banded((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
+    }
 
     const int8_t *__restrict__ bp = (const int8_t *)a.op0;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
@@ -307,6 +317,11 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
 #pragma unroll
             for (int y = 0; y < NB; ++y) g2[x][y] = g1[x][y];
     }
+    if constexpr (ADJ) {
+        pair::adjoint_epilogue<TILE>(a, it, L, g1, g2, rv, cv, pair::RowOf<float>{},
+                                     pair::NoCorr{}, prod == 0);
+        return;
+    }
     // The energy terms (pair::energy_epilogue's, with the virtual energies
     // from LDS); waves 4-7 join the block reduction with nothing to add.
     const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
@@ -345,9 +360,26 @@ hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st) {
     if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad || a.nvir_pad % 16 != 0)
         return hipErrorInvalidValue;
     if (a.tile == 64)
-        hipLaunchKernelGGL(energy_i8x4_kernel<64>, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
+        hipLaunchKernelGGL((energy_i8x4_kernel<64, false>), dim3((unsigned)a.nitems), dim3(512), 0,
+                           st, a);
     else if (a.tile == 32)
-        hipLaunchKernelGGL(energy_i8x4_kernel<32>, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
+        hipLaunchKernelGGL((energy_i8x4_kernel<32, false>), dim3((unsigned)a.nitems), dim3(512), 0,
+                           st, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_i8x4(const EnergyArgs &a, hipStream_t st) {
+    if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad || a.nvir_pad % 16 != 0 ||
+        a.i_fixed < 0 || !a.t_f32)
+        return hipErrorInvalidValue;
+    if (a.tile == 64)
+        hipLaunchKernelGGL((energy_i8x4_kernel<64, true>), dim3((unsigned)a.nitems), dim3(512), 0,
+                           st, a);
+    else if (a.tile == 32)
+        hipLaunchKernelGGL((energy_i8x4_kernel<32, true>), dim3((unsigned)a.nitems), dim3(512), 0,
+                           st, a);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

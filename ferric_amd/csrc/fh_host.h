@@ -97,6 +97,8 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t nau
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec);
 int tuned_tile(Prec prec, size_t nvir, size_t naux);
 size_t packed_elem_bytes(Prec p);
+// Bytes of one packed operand array of plan pl (INT8X4 adds TILE spare rows).
+size_t packed_op_bytes(const EnergyPlan &pl, size_t nocc, Prec prec);
 // neg_dev (naux_neg rows, same layout and dtype as b_dev) makes the kernel
 // signed: K = B^T B - N^T N (FP64 / FP32 only).
 fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux, size_t nocc,

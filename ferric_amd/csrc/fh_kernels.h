@@ -166,6 +166,8 @@ struct GemmArgs {
 hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st);
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st);
+// Phase A of the adjoint on the INT8 digit-slice kernel (T~ stored FP32).
+hipError_t launch_adjoint_i8x4(const EnergyArgs &a, hipStream_t st);
 // Opposite-spin rectangle (F64 / F32) and its row reduction + fold.
 hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_rect_reduce(const EnergyArgs &a, hipStream_t st);

@@ -140,14 +140,16 @@ __device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double 
 // Adjoint epilogue (phase A): energy partials as above, T~ written to the
 // row's scratch blocks, and per-item sums of t*T~ (by row a, by column b,
 // total) for dE/deps. Padding elements (nvir <= a, b < nvir_pad) of T~ are
-// written as zeros: the phase-B/C GEMMs contract over them.
+// written as zeros: the phase-B/C GEMMs contract over them. The 2x2 waves
+// holding the tile pass active = true; further waves of a larger workgroup
+// (the INT8 kernel's) pass false and only join the barriers and the block sum.
 template <int TILE, typename Acc, typename RowOfT, typename Corr>
 __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item &it,
                                                  const Lane &L,
                                                  Acc (&acc1)[TILE / 32][TILE / 32],
                                                  Acc (&acc2)[TILE / 32][TILE / 32],
                                                  const bool *rv, const bool *cv, RowOfT row_of,
-                                                 Corr corr) {
+                                                 Corr corr, bool active = true) {
     constexpr int TW = TILE / 2, NB = TW / 16;
     const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
     const double cd = a.c_os + a.c_ss, ce = a.c_ss;
@@ -167,7 +169,7 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
     for (int x = 0; x < NB; ++x)
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
-            if (!(rv[x] && cv[y])) continue;
+            if (!(active && rv[x] && cv[y])) continue;
             const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
             const bool bok = bb < a.nvir;
             const double eb = bok ? a.eps_vir[bb] : 0.0;
@@ -222,14 +224,14 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
             s += __shfl_xor(s, 2, kWave);
             s += __shfl_xor(s, 4, kWave);
             s += __shfl_xor(s, 8, kWave);
-            if (L.r == 0) rows_lds[L.wc][L.wr * TW + x * 16 + row_of(L.g, v)] = s;
+            if (active && L.r == 0) rows_lds[L.wc][L.wr * TW + x * 16 + row_of(L.g, v)] = s;
         }
 #pragma unroll
     for (int y = 0; y < NB; ++y) {
         double s = csum[y];
         s += __shfl_xor(s, 16, kWave);
         s += __shfl_xor(s, 32, kWave);
-        if (L.g == 0) cols_lds[L.wr][L.wc * TW + y * 16 + L.r] = s;
+        if (active && L.g == 0) cols_lds[L.wr][L.wc * TW + y * 16 + L.r] = s;
     }
     // Blocks skipped above (beyond nvir_pad) leave their LDS slots unset:
     // clear those slots first in a separate pass is unnecessary because the

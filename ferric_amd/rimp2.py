@@ -151,7 +151,8 @@ def energy_adjoint(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
     -- and ``deps`` = dE/deps over the full orbital-energy vector (2 diag P on the
     active occupied and virtual blocks, 0 for frozen core). A device tensor
     ``b_ov`` gives a device FP64 ``gamma``; a host array gives a numpy one.
-    Precision FP64 or BF16X3.
+    Precision FP64, BF16X3, or INT8X4 (the energy and T~ from the INT8 digit-slice
+    kernel, the gamma GEMMs in BF16x3).
     """
     if nocc_total is None:
         nocc_total = first_occ + nocc

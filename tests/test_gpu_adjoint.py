@@ -141,6 +141,40 @@ def test_bf16x3_adjoint_against_fp64(decane, decane_adj):
     assert math.fsum((p.b_ov * gam).ravel()) == pytest.approx(4 * e, rel=1e-5)
 
 
+def test_int8x4_adjoint_against_fp64(decane, decane_adj):
+    """INT8X4 adjoint: phase A on the INT8 digit-slice kernel, the gamma GEMMs in
+    BF16x3. Its energy is the INT8X4 energy (the same g, another summation
+    order) and within 1e-8 Eh of FP64; dE/deps comes from the FP64 epilogue on
+    that g; gamma has the BF16x3 GEMMs' accuracy."""
+    p = decane
+    _, e64, g64, d64 = decane_adj
+    sc, e, gam, deps = _adj(p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, 1.2, 1.0 / 3.0,
+                            Precision.INT8X4)
+    en = rimp2.spin_components(torch.from_numpy(p.b_ov).cuda(), p.eps, p.nocc, p.nvir,
+                               precision=Precision.INT8X4)
+    rel_g = np.max(np.abs(gam - g64)) / np.max(np.abs(g64))
+    rel_d = np.max(np.abs(deps - d64)) / np.max(np.abs(d64))
+    print(f"int8x4 adjoint: dE {e - e64:.3e}, max rel dGamma {rel_g:.2e}, dEps {rel_d:.2e}",
+          flush=True)
+    assert e == pytest.approx(1.2 * en.e_os + en.e_ss / 3.0, rel=1e-12)
+    assert abs(e - e64) <= 1e-8
+    assert rel_d <= 1e-7
+    assert rel_g <= 1e-4
+    assert math.fsum((p.b_ov * gam).ravel()) == pytest.approx(4 * e, rel=1e-5)
+
+
+def test_int8x4_adjoint_golden_frozen_core(golden):
+    for name in ("res_150_9_31_2_5", "water_shapes"):
+        c = golden[name]
+        args = (c["b"], c["eps"], c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"])
+        print("int8x4 adjoint", name, flush=True)
+        _, e64, g64, d64 = _adj(*args, 1.0, 1.0)
+        _, e8, g8, d8 = _adj(*args, 1.0, 1.0, Precision.INT8X4)
+        assert abs(e8 - e64) <= 1e-10 * max(1.0, abs(e64)), name
+        assert np.max(np.abs(d8 - d64)) <= 1e-7 * np.max(np.abs(d64)), name
+        assert np.max(np.abs(g8 - g64)) <= 1e-4 * np.max(np.abs(g64)), name
+
+
 def test_adjoint_deterministic(golden):
     c = golden["res_150_9_31_2_5"]
     args = (c["b"], c["eps"], c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"])
