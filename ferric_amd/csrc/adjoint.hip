@@ -20,7 +20,16 @@
 // Every Gamma element has exactly one owner per launch and launches are
 // stream-ordered, so the result is deterministic. T~ is never held for more
 // than one occupied row.
+//
+// INT8X4 (adjoint_core_i8): phase A runs once over every item on the INT8
+// digit-slice kernel (energy_i8.hip), which writes the whole T~ -- symmetric,
+// T~_ji^ba = T~_ij^ab -- as BF16 hi / lo over the padded (i, a) index space
+// (2 (nocc nvir_pad)^2 x 2 bytes: 33 GB at 100/900/3600, held in HBM); the
+// per-row reductions follow; then Gamma = 4 B T~ is one plain GEMM over that
+// space, handed to rocBLAS as the three products hi*hi + hi*lo + lo*hi with
+// FP32 accumulation (rocBLAS atomics off: deterministic) and widened to FP64.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -522,6 +531,51 @@ __global__ void adjoint_finish_kernel(const double *__restrict__ rows, int nocc,
     if (k < nvir) result[2 + nocc + k] = dvir_s[k] + dvir_c[k];
 }
 
+// BF16 hi/lo split of FP32 values (hi = RN(x), lo = RN(x - hi), the split of
+// pack.hip) over rows r < nr of cols elements each, row stride ld (in and out).
+__global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict__ x, int64_t nr,
+                                                         int64_t cols, int64_t ld,
+                                                         uint16_t *__restrict__ hi,
+                                                         uint16_t *__restrict__ lo) {
+    const int64_t n = nr * cols;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256) {
+        const int64_t idx = (k / cols) * ld + k % cols;
+        const float v = x[idx];
+        const __bf16 h = (__bf16)v;
+        const __bf16 l = (__bf16)(v - (float)h);
+        hi[idx] = __builtin_bit_cast(uint16_t, h);
+        lo[idx] = __builtin_bit_cast(uint16_t, l);
+    }
+}
+
+// gamma[Q][i*nvir + a] = x[Q][i*nvir_pad + a] (FP32 over the padded index
+// space -> FP64 over the reference layout).
+__global__ __launch_bounds__(256) void widen_kernel(const float *__restrict__ x, int naux,
+                                                    int nocc, int nvir, int nvir_pad,
+                                                    double *__restrict__ y) {
+    const int64_t nov = (int64_t)nocc * nvir, rows = (int64_t)nocc * nvir_pad;
+    const int64_t n = (int64_t)naux * nov;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256) {
+        const int64_t q = k / nov, c = k % nov;
+        y[k] = (double)x[q * rows + (c / nvir) * nvir_pad + c % nvir];
+    }
+}
+
+void launch_split(const float *x, int64_t nr, int64_t cols, int64_t ld, uint16_t *hi,
+                  uint16_t *lo, hipStream_t st) {
+    const int64_t n = nr * cols;
+    if (n <= 0) return;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(split_bf16_kernel, dim3(blocks), dim3(256), 0, st, x, nr, cols, ld, hi, lo);
+}
+
+fh_status blas_ok(rocblas_status st, const char *what) {
+    if (st == rocblas_status_success) return FH_OK;
+    return fail(FH_KERNEL, "%s: rocBLAS status %d", what, (int)st);
+}
+
 template <typename T>
 void launch_fill(T *p, int64_t n, T v, hipStream_t st) {
     int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
@@ -573,6 +627,216 @@ static int gemm_tile(Prec prec) {
     return 64;
 }
 
+static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux,
+                                 size_t nocc, size_t nvir, const double *eps, size_t n_eps,
+                                 size_t first_occ, size_t nocc_total, double c_os, double c_ss,
+                                 hipStream_t stream, fh_spin_components *out, double *gamma_dev,
+                                 double *deps) {
+    const Prec prec = Prec::I8X4;
+    const bool in_f32 = b_dtype == FH_DT_F32;
+    EnergyPlan pl = plan_energy(nocc, nvir, naux, prec);
+    if (pl.naux_pad > kI8MaxNauxPad)
+        return fail(FH_RANGE, "naux_pad %d beyond the INT8 slices' int32 range (%d)", pl.naux_pad,
+                    kI8MaxNauxPad);
+    const size_t rows = nocc * (size_t)pl.nvir_pad;
+    if (rows > (size_t)0x7fffffff)
+        return fail(FH_UNSUPPORTED, "nocc * nvir_pad = %zu beyond 32-bit GEMM sizes", rows);
+    const int vec_stride = 2 * pl.tile + 2;
+    const size_t items = (size_t)pl.nitems;
+    const int nvb = (int)((nvir + 255) / 256);
+    const int nchunks_max = (int)std::min<size_t>(nocc, (size_t)std::max(1, (512 + nvb - 1) / nvb));
+    std::vector<std::pair<int, size_t>> req = {
+        {kSlotOp0, packed_op_bytes(pl, nocc, prec)},
+        {kSlotScale, sizeof(int) * rows},
+        {kSlotStage2, sizeof(float) * naux * rows},
+        {kSlotB2Split, 2 * sizeof(uint16_t) * naux * rows},
+        {kSlotTSplit, 2 * sizeof(uint16_t) * rows * rows},
+        {kSlotG32, sizeof(float) * naux * rows},
+        {kSlotEps, sizeof(double) * (nocc + nvir)},
+        {kSlotItems, sizeof(double) * 4 * items},
+        {kSlotDeps, sizeof(double) * vec_stride * items},
+        {kSlotRows, sizeof(double) * 4 * nocc},
+        {kSlotAcc, sizeof(double) * 2 * (nocc + nvir)},
+        {kSlotResult, sizeof(double) * (2 + nocc + nvir)},
+        {kSlotScalar, 256},
+        {kSlotRed, sizeof(double) * (2 * (size_t)nchunks_max * nvir + 6 * nocc)},
+    };
+    fh_status st = reserve(s, req);
+    if (st != FH_OK) return st;
+    // B enters the GEMMs as FP32 split into BF16: the FP32 range check applies.
+    st = range_check(s, b_dev, in_f32, naux * nocc * nvir, naux, stream);
+    if (st != FH_OK) return st;
+    std::vector<double> e(nocc + nvir);
+    for (size_t i = 0; i < nocc; ++i) e[i] = eps[first_occ + i];
+    for (size_t a = 0; a < nvir; ++a) e[nocc + a] = eps[nocc_total + a];
+    double *deps_dev = (double *)s.slots[kSlotEps].ptr;
+    st = hip_check(hipMemcpyAsync(deps_dev, e.data(), sizeof(double) * e.size(),
+                                  hipMemcpyHostToDevice, stream),
+                   "H2D eps");
+    if (st != FH_OK) return st;
+    s.pending.h2d_bytes += sizeof(double) * e.size();
+
+    mark(s, 0, stream);
+    PackArgs pa;
+    pa.b = b_dev;
+    pa.in_f32 = in_f32;
+    pa.naux = (int)naux;
+    pa.nocc = (int)nocc;
+    pa.nvir = (int)nvir;
+    pa.nvir_pad = pl.nvir_pad;
+    pa.naux_pad = pl.naux_pad;
+    pa.fmt = PackFmt::I8X4;
+    pa.out0 = s.slots[kSlotOp0].ptr;
+    pa.out1 = nullptr;
+    pa.scale = nullptr;
+    pa.lo_sq = nullptr;
+    pa.rexp = (int *)s.slots[kSlotScale].ptr;
+    hipError_t he = launch_pack(pa, stream);
+    if (he != hipSuccess) return fail(FH_KERNEL, "pack kernel: %s", hipGetErrorString(he));
+    float *b2 = (float *)s.slots[kSlotStage2].ptr;
+    {
+        const int64_t total = (int64_t)naux * (int64_t)rows;
+        const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 16384);
+        if (in_f32)
+            hipLaunchKernelGGL((pack_qmajor_kernel<float, float>), dim3(blocks), dim3(256), 0,
+                               stream, (const float *)b_dev, (int)naux, (int)nocc, (int)nvir,
+                               pl.nvir_pad, (int)naux, b2);
+        else
+            hipLaunchKernelGGL((pack_qmajor_kernel<double, float>), dim3(blocks), dim3(256), 0,
+                               stream, (const double *)b_dev, (int)naux, (int)nocc, (int)nvir,
+                               pl.nvir_pad, (int)naux, b2);
+    }
+    uint16_t *b2hi = (uint16_t *)s.slots[kSlotB2Split].ptr, *b2lo = b2hi + naux * rows;
+    uint16_t *thi = (uint16_t *)s.slots[kSlotTSplit].ptr, *tlo = thi + rows * rows;
+    float *x32 = (float *)s.slots[kSlotG32].ptr;
+    launch_split(b2, (int64_t)naux, (int64_t)rows, (int64_t)rows, b2hi, b2lo, stream);
+    double *acc = (double *)s.slots[kSlotAcc].ptr;
+    launch_fill<double>(acc, (int64_t)(2 * (nocc + nvir)), 0.0, stream);
+    he = hipGetLastError();
+    if (he != hipSuccess) return fail(FH_KERNEL, "adjoint set-up kernels: %s", hipGetErrorString(he));
+    mark(s, 1, stream);
+
+    // Phase A: every item in one launch; T~ into the whole-T~ matrix.
+    EnergyArgs ea;
+    ea.prec = prec;
+    ea.tile = pl.tile;
+    ea.op0 = pa.out0;
+    ea.op1 = nullptr;
+    ea.scale = nullptr;
+    ea.diag_corr = nullptr;
+    ea.rexp = pa.rexp;
+    ea.nocc = (int)nocc;
+    ea.nvir = (int)nvir;
+    ea.nvir_pad = pl.nvir_pad;
+    ea.naux_pad = pl.naux_pad;
+    ea.nqb = 0;
+    ea.eps_occ = deps_dev;
+    ea.eps_vir = deps_dev + nocc;
+    ea.kappa = NAN;
+    ea.ntile = pl.ntile;
+    ea.ntp = pl.ntp;
+    ea.nitems = pl.nitems;
+    ea.item_partials = (double *)s.slots[kSlotItems].ptr;
+    ea.row_partials = (double *)s.slots[kSlotRows].ptr;
+    ea.result = (double *)s.slots[kSlotResult].ptr;
+    ea.c_os = c_os;
+    ea.c_ss = c_ss;
+    ea.item_vec = (double *)s.slots[kSlotDeps].ptr;
+    ea.vec_stride = vec_stride;
+    ea.i_fixed = -1;
+    ea.tfull_hi = thi;
+    ea.tfull_lo = tlo;
+    ea.ld_full = (int64_t)rows;
+    he = launch_adjoint_pairs(ea, stream);
+    if (he != hipSuccess) return fail(FH_KERNEL, "adjoint pair kernel: %s", hipGetErrorString(he));
+
+    // Per-row reductions of the energy and dE/deps (each row's items are the
+    // contiguous pairs (i, j >= i)).
+    AdjRed rd;
+    rd.nocc = (int)nocc;
+    rd.nvir = (int)nvir;
+    rd.tile = pl.tile;
+    rd.ntile = pl.ntile;
+    rd.ntp = pl.ntp;
+    rd.vec_stride = vec_stride;
+    rd.rows = ea.row_partials;
+    rd.docc_s = acc;
+    rd.docc_c = acc + nocc;
+    rd.dvir_s = acc + 2 * nocc;
+    rd.dvir_c = acc + 2 * nocc + nvir;
+    rd.vpart = (double *)s.slots[kSlotRed].ptr;
+    rd.ppart = rd.vpart + 2 * (size_t)nchunks_max * nvir;
+    for (size_t i = 0; i < nocc; ++i) {
+        const int64_t first = ((int64_t)i * (int64_t)nocc - (int64_t)i * ((int64_t)i - 1) / 2) * pl.ntp;
+        rd.i = (int)i;
+        rd.item_partials = ea.item_partials + 4 * first;
+        rd.item_vec = ea.item_vec + first * vec_stride;
+        const int npair = (int)(nocc - i);
+        rd.nchunks = std::min(npair, nchunks_max);
+        rd.jc = (npair + rd.nchunks - 1) / rd.nchunks;
+        rd.nchunks = (npair + rd.jc - 1) / rd.jc;
+        hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)(nvb * rd.nchunks + npair)),
+                           dim3(256), 0, stream, rd);
+        hipLaunchKernelGGL(adjoint_fold_kernel, dim3((unsigned)nvb), dim3(256), 0, stream, rd);
+    }
+    he = hipGetLastError();
+    if (he != hipSuccess) return fail(FH_KERNEL, "adjoint reduction: %s", hipGetErrorString(he));
+
+    // Gamma = 4 B T~ (column-major for rocBLAS: X^T (rows x naux) = T~ (rows x
+    // rows, symmetric) x B^T (rows x naux)), three BF16 products.
+    {
+        void *hv = nullptr;
+        st = blas_handle(s, stream, &hv);
+        if (st != FH_OK) return st;
+        const float alpha = 4.0f, zero = 0.0f, one = 1.0f;
+        const uint16_t *tp[3] = {thi, thi, tlo}, *bp[3] = {b2hi, b2lo, b2hi};
+        const int r = (int)rows;
+        for (int p = 0; p < 3; ++p) {
+            st = blas_ok(rocblas_gemm_ex((rocblas_handle)hv, rocblas_operation_none,
+                                         rocblas_operation_none, r, (int)naux, r, &alpha, tp[p],
+                                         rocblas_datatype_bf16_r, r, bp[p], rocblas_datatype_bf16_r,
+                                         r, p == 0 ? &zero : &one, x32, rocblas_datatype_f32_r, r,
+                                         x32, rocblas_datatype_f32_r, r, rocblas_datatype_f32_r,
+                                         rocblas_gemm_algo_standard, 0, 0),
+                         "adjoint gamma GEMM");
+            if (st != FH_OK) return st;
+        }
+        const int64_t n = (int64_t)naux * (int64_t)(nocc * nvir);
+        const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
+        hipLaunchKernelGGL(widen_kernel, dim3(blocks), dim3(256), 0, stream, x32, (int)naux,
+                           (int)nocc, (int)nvir, pl.nvir_pad, gamma_dev);
+    }
+    mark(s, 2, stream);
+    {
+        const int n = (int)std::max(nocc, nvir);
+        hipLaunchKernelGGL(adjoint_finish_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           stream, ea.row_partials, (int)nocc, (int)nvir, rd.docc_s, rd.docc_c,
+                           rd.dvir_s, rd.dvir_c, ea.result);
+    }
+    he = hipGetLastError();
+    if (he != hipSuccess) return fail(FH_KERNEL, "adjoint finish: %s", hipGetErrorString(he));
+    mark(s, 3, stream);
+    s.pending.kernel_launches += 2 * nocc + 12;
+    std::vector<double> res(2 + nocc + nvir);
+    st = hip_check(hipMemcpyAsync(res.data(), ea.result, sizeof(double) * res.size(),
+                                  hipMemcpyDeviceToHost, stream),
+                   "D2H adjoint result");
+    if (st != FH_OK) return st;
+    st = hip_check(hipStreamSynchronize(stream), "stream sync");
+    if (st != FH_OK) return st;
+    s.pending.d2h_bytes += sizeof(double) * res.size();
+    finish_timing(s, (uint64_t)pl.nitems, pl.tile);
+    out->e_os = res[0];
+    out->e_ss = res[1];
+    out->e_total = res[0] + res[1];
+    if (deps) {
+        for (size_t k = 0; k < n_eps; ++k) deps[k] = 0.0;
+        for (size_t i = 0; i < nocc; ++i) deps[first_occ + i] = res[2 + i];
+        for (size_t a = 0; a < nvir; ++a) deps[nocc_total + a] = res[2 + nocc + a];
+    }
+    return FH_OK;
+}
+
 static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux,
                               size_t nocc, size_t nvir, const double *eps, size_t n_eps,
                               size_t first_occ, size_t nocc_total, double c_os, double c_ss,
@@ -580,14 +844,11 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
                               double *gamma_dev, double *deps) {
     const bool in_f32 = b_dtype == FH_DT_F32;
     const bool f64 = prec == Prec::F64;
-    // INT8X4: phase A on the INT8 digit-slice kernel (the energy and T~ with
-    // FP64-class accuracy), phases B/C on the BF16x3 GEMMs (T~ and B kept FP32).
-    const bool i8 = prec == Prec::I8X4;
+    if (prec == Prec::I8X4)
+        return adjoint_core_i8(s, b_dev, b_dtype, naux, nocc, nvir, eps, n_eps, first_occ,
+                               nocc_total, c_os, c_ss, stream, out, gamma_dev, deps);
     const Prec gprec = f64 ? Prec::F64 : Prec::BF16X3;
     EnergyPlan pl = plan_energy(nocc, nvir, naux, prec);
-    if (i8 && pl.naux_pad > kI8MaxNauxPad)
-        return fail(FH_RANGE, "naux_pad %d beyond the INT8 slices' int32 range (%d)", pl.naux_pad,
-                    kI8MaxNauxPad);
     // The FP64 adjoint epilogue does not fit a 128-wide tile in the register
     // file (it would spill); 64-wide tiles run two workgroups per CU.
     if (f64 && pl.tile > 64) pl.tile = 64;
@@ -637,7 +898,6 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
         req.push_back({kSlotOp1, op_bytes});
         req.push_back({kSlotDiag, sizeof(double) * rows});
     }
-    if (i8) req.push_back({kSlotScale, sizeof(int) * rows});
     fh_status st = reserve(s, req);
     if (st != FH_OK) return st;
     if (!f64) {
@@ -663,12 +923,11 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     pa.nvir = (int)nvir;
     pa.nvir_pad = pl.nvir_pad;
     pa.naux_pad = pl.naux_pad;
-    pa.fmt = f64 ? PackFmt::F64 : i8 ? PackFmt::I8X4 : PackFmt::BF16X2;
+    pa.fmt = f64 ? PackFmt::F64 : PackFmt::BF16X2;
     pa.out0 = s.slots[kSlotOp0].ptr;
     pa.out1 = prec == Prec::BF16X3 ? s.slots[kSlotOp1].ptr : nullptr;
     pa.scale = nullptr;
     pa.lo_sq = prec == Prec::BF16X3 ? (double *)s.slots[kSlotDiag].ptr : nullptr;
-    pa.rexp = i8 ? (int *)s.slots[kSlotScale].ptr : nullptr;
     hipError_t he = launch_pack(pa, stream);
     if (he != hipSuccess) return fail(FH_KERNEL, "pack kernel: %s", hipGetErrorString(he));
     {
@@ -709,7 +968,6 @@ static fh_status adjoint_core(State &s, const void *b_dev, fh_dtype b_dtype, siz
     ea.op1 = pa.out1;
     ea.scale = nullptr;
     ea.diag_corr = pa.lo_sq;
-    ea.rexp = pa.rexp;
     ea.nocc = (int)nocc;
     ea.nvir = (int)nvir;
     ea.nvir_pad = pl.nvir_pad;

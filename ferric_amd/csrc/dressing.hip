@@ -24,18 +24,31 @@ fh_status blas_check(rocblas_status st, const char *what) {
     return fail(FH_KERNEL, "%s: rocBLAS status %d", what, (int)st);
 }
 
-fh_status dress_core(State &s, const double *j3, size_t naux, size_t nao, const double *cl,
-                     size_t nleft, const double *cr, size_t nright, const double *m, size_t nband,
-                     hipStream_t stream, double *out) {
+}  // namespace
+
+fh_status blas_handle(State &s, hipStream_t stream, void **handle) {
     if (!s.blas) {
         rocblas_handle h = nullptr;
         fh_status st = blas_check(rocblas_create_handle(&h), "rocblas_create_handle");
         if (st != FH_OK) return st;
+        st = blas_check(rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed),
+                        "rocblas_set_atomics_mode");
+        if (st != FH_OK) return st;
         s.blas = h;
     }
-    rocblas_handle h = (rocblas_handle)s.blas;
-    fh_status st = blas_check(rocblas_set_stream(h, stream), "rocblas_set_stream");
+    *handle = s.blas;
+    return blas_check(rocblas_set_stream((rocblas_handle)s.blas, stream), "rocblas_set_stream");
+}
+
+namespace {
+
+fh_status dress_core(State &s, const double *j3, size_t naux, size_t nao, const double *cl,
+                     size_t nleft, const double *cr, size_t nright, const double *m, size_t nband,
+                     hipStream_t stream, double *out) {
+    void *hv = nullptr;
+    fh_status st = blas_handle(s, stream, &hv);
     if (st != FH_OK) return st;
+    rocblas_handle h = (rocblas_handle)hv;
     const size_t width = nleft * nright;
     st = reserve(s, {{kSlotDressX, sizeof(double) * naux * nao * nleft},
                      {kSlotDressY, sizeof(double) * naux * width}});

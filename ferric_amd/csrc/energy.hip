@@ -180,7 +180,9 @@ hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_adjoint_pairs(const EnergyArgs &a, hipStream_t st) {
-    if (a.nitems <= 0 || a.nitems > 0x7fffffffLL || a.i_fixed < 0) return hipErrorInvalidValue;
+    if (a.nitems <= 0 || a.nitems > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (a.prec == Prec::I8X4) return launch_adjoint_i8x4(a, st);
+    if (a.i_fixed < 0) return hipErrorInvalidValue;
     hipError_t e = hipErrorInvalidValue;
     switch (a.prec) {
     case Prec::F64:

@@ -106,9 +106,9 @@ __device__ __forceinline__ double combine_levels(int s2, int s3, int s4, int s5,
 // halves meet through LDS after the loop (an exact int32 merge of the level
 // sums for a single-product item; FP64 C2 for a two-product item), and waves
 // 0-3 run the energy epilogue from LDS-staged orbital energies and row
-// exponents. ADJ: phase A of the adjoint for occupied row a.i_fixed (items
-// (i_fixed, j >= i_fixed) x tile pairs), ending in the T~ epilogue of
-// pair_body.h instead of the energy terms.
+// exponents. ADJ: phase A of the adjoint over every item at once, ending in
+// the T~ epilogue of pair_body.h (writing the whole-T~ matrix) instead of the
+// energy terms.
 template <int TILE, bool ADJ>
 __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
     constexpr int TW = TILE / 2, NB = TW / 16;
@@ -122,16 +122,11 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
 
     pair::Item it;
     it.id = pair_interleave((int64_t)blockIdx.x, a.ntp, a.nitems / a.ntp);
-    if (ADJ) {
-        // the adjoint's row reduction reads each item's row / column sums at
-        // item id (j - i) * ntp + t with t in decode_tile_pair's order
-        it.i = a.i_fixed;
-        it.j = a.i_fixed + (int)(it.id / a.ntp);
+    decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
+    if (ADJ)  // the adjoint's row reduction reads each item's sums in decode_tile_pair's order
         decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
-    } else {
-        decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
+    else
         decode_tile_pair_banded((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
-    }
 
     const int8_t *__restrict__ bp = (const int8_t *)a.op0;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
@@ -318,8 +313,8 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
             for (int y = 0; y < NB; ++y) g2[x][y] = g1[x][y];
     }
     if constexpr (ADJ) {
-        pair::adjoint_epilogue<TILE>(a, it, L, g1, g2, rv, cv, pair::RowOf<float>{},
-                                     pair::NoCorr{}, prod == 0);
+        pair::adjoint_epilogue<TILE, v4d, pair::RowOf<float>, pair::NoCorr, true>(
+            a, it, L, g1, g2, rv, cv, pair::RowOf<float>{}, pair::NoCorr{}, prod == 0);
         return;
     }
     // The energy terms (pair::energy_epilogue's, with the virtual energies
@@ -372,7 +367,7 @@ hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st) {
 
 hipError_t launch_adjoint_i8x4(const EnergyArgs &a, hipStream_t st) {
     if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad || a.nvir_pad % 16 != 0 ||
-        a.i_fixed < 0 || !a.t_f32)
+        a.i_fixed >= 0 || !a.tfull_hi || !a.tfull_lo || a.ld_full != (int64_t)a.nocc * a.nvir_pad)
         return hipErrorInvalidValue;
     if (a.tile == 64)
         hipLaunchKernelGGL((energy_i8x4_kernel<64, true>), dim3((unsigned)a.nitems), dim3(512), 0,

@@ -35,6 +35,9 @@ enum SlotId {
     kSlotRed,        // adjoint: per-row reduction partials
     kSlotDressX,     // dressing: (Q|nu i) half-transformed rows
     kSlotDressY,     // dressing: undressed MO rows (Q|ia)
+    kSlotB2Split,    // INT8X4 adjoint: BF16 hi and lo of the Q-major copy of B
+    kSlotTSplit,     // INT8X4 adjoint: BF16 hi and lo of the whole symmetric T~
+    kSlotG32,        // INT8X4 adjoint: FP32 X = B T~ of the rocBLAS GEMMs
     kNumSlots
 };
 
@@ -59,7 +62,7 @@ struct State {
     fh_stats stats{};
     Pending pending;
     GraphCache *graphs = nullptr;
-    void *blas = nullptr;  // rocBLAS handle of the device (dressing.hip), made on first use
+    void *blas = nullptr;  // rocBLAS handle of the device, made on first use (blas_handle)
     // Phase timing (FERRIC_TIMING): events on the call's stream.
     int timing = -1;  // -1: not yet read from the environment
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -97,6 +100,9 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t nau
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec);
 int tuned_tile(Prec prec, size_t nvir, size_t naux);
 size_t packed_elem_bytes(Prec p);
+// The device's rocBLAS handle (created on first use, atomics off so that its
+// GEMMs are deterministic), bound to stream.
+fh_status blas_handle(State &s, hipStream_t stream, void **handle);
 // Bytes of one packed operand array of plan pl (INT8X4 adds TILE spare rows).
 size_t packed_op_bytes(const EnergyPlan &pl, size_t nocc, Prec prec);
 // neg_dev (naux_neg rows, same layout and dtype as b_dev) makes the kernel

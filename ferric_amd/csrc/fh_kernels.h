@@ -130,6 +130,11 @@ struct EnergyArgs {
     bool t_f32 = false;       // T stored FP32 (reduced-precision adjoint) or FP64
     double *item_vec = nullptr;  // per item: row sums, column sums, total of t*T~
     int vec_stride = 0;
+    // Whole-T~ adjoint (INT8X4): T~ as BF16 hi / lo over the padded (i, a) index
+    // space, T~[i*nvir_pad + a][j*nvir_pad + b] = T~_ij^ab for every i, j (the
+    // matrix is symmetric), row stride ld_full.
+    uint16_t *tfull_hi = nullptr, *tfull_lo = nullptr;
+    int64_t ld_full = 0;
     // Signed factor (K = B+^T B+ - B-^T B-): packed Q columns from q_neg on
     // hold B- and enter with -1.
     int q_neg = 1 << 30;
@@ -166,7 +171,8 @@ struct GemmArgs {
 hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st);
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st);
-// Phase A of the adjoint on the INT8 digit-slice kernel (T~ stored FP32).
+// Phase A of the adjoint on the INT8 digit-slice kernel, every item in one
+// launch (i_fixed < 0), T~ into the whole-T~ matrix.
 hipError_t launch_adjoint_i8x4(const EnergyArgs &a, hipStream_t st);
 // Opposite-spin rectangle (F64 / F32) and its row reduction + fold.
 hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st);

@@ -130,6 +130,17 @@ __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item 
     block_store(os, ss, w, a.item_partials + 4 * it.id);
 }
 
+// One element of the whole-T~ matrix as BF16 hi = RN(x), lo = RN(x - hi) of
+// x = RN_f32(v) (the split of pack.hip).
+__device__ __forceinline__ void tfull_store(const EnergyArgs &a, int64_t r, int64_t c, double v) {
+    const float x = (float)v;
+    const __bf16 h = (__bf16)x;
+    const __bf16 l = (__bf16)(x - (float)h);
+    const int64_t k = r * a.ld_full + c;
+    a.tfull_hi[k] = __builtin_bit_cast(uint16_t, h);
+    a.tfull_lo[k] = __builtin_bit_cast(uint16_t, l);
+}
+
 __device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double v) {
     if (f32)
         ((float *)buf)[idx] = (float)v;
@@ -143,7 +154,9 @@ __device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double 
 // written as zeros: the phase-B/C GEMMs contract over them. The 2x2 waves
 // holding the tile pass active = true; further waves of a larger workgroup
 // (the INT8 kernel's) pass false and only join the barriers and the block sum.
-template <int TILE, typename Acc, typename RowOfT, typename Corr>
+// FULLT: T~ goes to the whole-T~ matrix (a.tfull_*) at (i a, j b), (i b, j a)
+// and their mirrors instead of the row blocks tbuf / ubuf.
+template <int TILE, typename Acc, typename RowOfT, typename Corr, bool FULLT = false>
 __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item &it,
                                                  const Lane &L,
                                                  Acc (&acc1)[TILE / 32][TILE / 32],
@@ -196,14 +209,26 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
                     rsum[x][v] += val;
                     csum[y] += val;
                 }
-                // T~_ij^ab and T~_ij^ba into the row-i blocks.
-                tstore(a.tbuf, a.t_f32, (int64_t)aa * a.ld_t + (int64_t)jj * a.nvir_pad + bb, T1);
-                if (!diag_tile)
-                    tstore(a.tbuf, a.t_f32, (int64_t)bb * a.ld_t + (int64_t)jj * a.nvir_pad + aa,
-                           T2);
-                if (jj > 0) {
-                    tstore(a.ubuf, a.t_f32, ub + (int64_t)bb * a.nvir_pad + aa, T1);
-                    if (!diag_tile) tstore(a.ubuf, a.t_f32, ub + (int64_t)aa * a.nvir_pad + bb, T2);
+                if constexpr (FULLT) {
+                    const int64_t ri = (int64_t)it.i * a.nvir_pad, rj = (int64_t)it.j * a.nvir_pad;
+                    tfull_store(a, ri + aa, rj + bb, T1);
+                    tfull_store(a, rj + bb, ri + aa, T1);
+                    if (!diag_tile) {
+                        tfull_store(a, ri + bb, rj + aa, T2);
+                        tfull_store(a, rj + aa, ri + bb, T2);
+                    }
+                } else {
+                    // T~_ij^ab and T~_ij^ba into the row-i blocks.
+                    tstore(a.tbuf, a.t_f32,
+                           (int64_t)aa * a.ld_t + (int64_t)jj * a.nvir_pad + bb, T1);
+                    if (!diag_tile)
+                        tstore(a.tbuf, a.t_f32,
+                               (int64_t)bb * a.ld_t + (int64_t)jj * a.nvir_pad + aa, T2);
+                    if (jj > 0) {
+                        tstore(a.ubuf, a.t_f32, ub + (int64_t)bb * a.nvir_pad + aa, T1);
+                        if (!diag_tile)
+                            tstore(a.ubuf, a.t_f32, ub + (int64_t)aa * a.nvir_pad + bb, T2);
+                    }
                 }
             }
         }
