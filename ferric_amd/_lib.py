@@ -140,6 +140,10 @@ SIGNATURES = {
                                              ctypes.POINTER(SpinComponentsC), _c_vp, _c_dp]),
     "fh_rimp2_batched_dev": (_c_int, [ctypes.POINTER(DomainC), _c_sz, _c_vp, _c_sz, _c_vp,
                                       _c_sz, _c_int, _c_int, _c_vp, _c_dp, _c_dp]),
+    "fh_rimp2_pair_energies": (_c_int, [_c_dp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz, _c_sz,
+                                        _c_int, _c_dp, _c_dp]),
+    "fh_rimp2_pair_energies_dev": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz,
+                                            _c_sz, _c_sz, _c_int, _c_vp, _c_dp, _c_dp]),
     "fh_dress_b_ov_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz,
                                    _c_vp, _c_vp]),
     "fh_u_same_spin": (_c_int, [ctypes.POINTER(SpinChannelC), _c_sz, _c_int, _c_dp]),
@@ -158,7 +162,7 @@ SIGNATURES = {
 }
 
 _lib = None
-ABI_VERSION = 4   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 5   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

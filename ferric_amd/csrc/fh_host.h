@@ -38,6 +38,7 @@ enum SlotId {
     kSlotB2Split,    // INT8X4 adjoint: BF16 hi and lo of the Q-major copy of B
     kSlotTSplit,     // INT8X4 adjoint: BF16 hi and lo of the whole symmetric T~
     kSlotG32,        // INT8X4 adjoint: FP32 X = B T~ of the rocBLAS GEMMs
+    kSlotPairs,      // pair energies: (e_os, e_ss) per unique pair
     kNumSlots
 };
 

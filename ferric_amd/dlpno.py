@@ -1,0 +1,204 @@
+"""Local (DLPNO) MP2 pair layer on the device: pair energies and the pair screen.
+
+Mirrors the pair-driven path of crates/ferric-mp2/src/dlpno_mp2.rs and
+crates/ferric-mp2/src/pair_energy_screen.rs:
+
+* :func:`estimate_pair_energies_from_b_ov` (dlpno_mp2.rs:527-594): the MP2
+  energy of every occupied pair from one pass of the fused pair kernel
+  (``fh_rimp2_pair_energies[_dev]``), any precision of the energy;
+* :class:`DlpnoConfig` (dlpno_mp2.rs:86-170) and
+  :func:`build_pair_domains_by_energy` (pair_energy_screen.rs:206-282): pairs
+  ``i <= j`` kept when ``|e_ij| >= t_cut_pairs`` (diagonals always), and the
+  geometric coupling lists on pair centroids;
+* :func:`dlpno_mp2_from_b_ov` (dlpno_mp2.rs:670-760) for ``t_cut_pno == 0``.
+  There every pair's PNO transform is a square orthogonal rotation, and the
+  semicanonical rotation after it re-diagonalises the virtual Fock block, so
+  both cancel out of the pair energy (the reference's exactness contract,
+  dlpno_mp2.rs:797-830): the DLPNO energy is the sum of the retained pairs'
+  canonical energies, which the device computes. PNO truncation
+  (``t_cut_pno > 0``) is not on the device and raises UnsupportedError, for the
+  caller's CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import LayoutError, Precision, UnsupportedError, check, lib
+from .rimp2 import SpinComponents, _is_torch_cuda
+
+DEFAULT_T_CUT_PAIRS = 1e-5  # dlpno_mp2.rs:121
+
+
+@dataclass
+class DlpnoConfig:
+    """dlpno_mp2.rs:86-170. Default: pair-energy screening at 1e-5 Eh, PNO and
+    distance screens off."""
+
+    pair_cutoff: float = math.inf
+    coupling_cutoff: float = math.inf
+    t_cut_pno: float = 0.0
+    t_cut_pairs: float = DEFAULT_T_CUT_PAIRS
+
+    @staticmethod
+    def exact() -> "DlpnoConfig":
+        return DlpnoConfig(math.inf, math.inf, 0.0, 0.0)
+
+    def is_exact(self) -> bool:
+        return math.isinf(self.pair_cutoff) and self.t_cut_pno == 0.0 and self.t_cut_pairs == 0.0
+
+
+@dataclass
+class PairEnergies:
+    """pair_energy_screen.rs:65-90: ``e`` is the symmetric nocc x nocc matrix of
+    pair energies (mirror factor applied); ``e_os`` + ``e_ss`` = ``e``."""
+
+    nocc: int
+    e: np.ndarray
+    e_os: np.ndarray
+    e_ss: np.ndarray
+
+    def e_ij(self, i: int, j: int) -> float:
+        return float(self.e[min(i, j), max(i, j)])
+
+    def total(self) -> float:
+        return float(sum(self.e[i, j] for i in range(self.nocc) for j in range(i, self.nocc)))
+
+    def discarded_energy(self, t_cut_pairs: float) -> float:
+        """pair_energy_screen.rs:90-105: the energy of the off-diagonal pairs a
+        threshold drops (|e_ij| < t_cut_pairs)."""
+        return float(sum(self.e[i, j] for i in range(self.nocc) for j in range(i + 1, self.nocc)
+                         if abs(self.e[i, j]) < t_cut_pairs))
+
+
+@dataclass
+class PairDomains:
+    """pair_domains.rs:51-65 (the fields the MP2 layer uses)."""
+
+    nocc: int
+    pairs: list
+    coupled: list
+    centers: np.ndarray
+    pair_cutoff: float = math.inf
+    coupling_cutoff: float = math.inf
+
+    def pair_retention(self) -> float:
+        """pair_domains.rs:68-75."""
+        total = self.nocc * (self.nocc + 1) // 2
+        return 1.0 if total == 0 else len(self.pairs) / total
+
+    def coupling_retention(self) -> float:
+        """pair_domains.rs:77-88."""
+        total = len(self.pairs) ** 2
+        return 1.0 if total == 0 else sum(len(c) for c in self.coupled) / total
+
+    def is_complete(self) -> bool:
+        """pair_domains.rs:90-98."""
+        n = len(self.pairs)
+        return n == self.nocc * (self.nocc + 1) // 2 and all(len(c) == n for c in self.coupled)
+
+
+@dataclass
+class DlpnoDiagnostics:
+    """dlpno_mp2.rs:174-185."""
+
+    pair_retention: float
+    virtual_retention: float
+    max_discarded_weight: float
+    n_pairs: int
+    extra: dict = field(default_factory=dict)
+
+
+def estimate_pair_energies_from_b_ov(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
+                                     nocc_total: int | None = None,
+                                     precision: Precision = Precision.FP64) -> PairEnergies:
+    """dlpno_mp2.rs:527-594 on the device. ``b_ov``: numpy array (host entry) or
+    contiguous CUDA tensor (device entry on the current torch stream)."""
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    e_os = np.zeros((nocc, nocc))
+    e_ss = np.zeros((nocc, nocc))
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    if _is_torch_cuda(b_ov):
+        import torch
+
+        if b_ov.dim() != 2 or not b_ov.is_contiguous():
+            raise LayoutError("b_ov must be a contiguous 2-D (naux, nocc*nvir) tensor")
+        if b_ov.dtype not in (torch.float64, torch.float32):
+            raise LayoutError(f"b_ov dtype {b_ov.dtype} is not float64/float32")
+        if b_ov.shape[1] != nocc * nvir:
+            raise LayoutError(f"b_ov is {tuple(b_ov.shape)}, expected width {nocc * nvir}")
+        dt = 0 if b_ov.dtype == torch.float64 else 1
+        stream = torch.cuda.current_stream(b_ov.device).cuda_stream
+        st = L.fh_rimp2_pair_energies_dev(
+            ctypes.c_void_p(b_ov.data_ptr()), dt, b_ov.shape[0], nocc, nvir, e.ctypes.data_as(dp),
+            e.size, first_occ, nocc_total, int(precision), ctypes.c_void_p(stream),
+            e_os.ctypes.data_as(dp), e_ss.ctypes.data_as(dp))
+    else:
+        b = np.ascontiguousarray(np.asarray(b_ov, dtype=np.float64))
+        if b.ndim != 2:
+            raise LayoutError("b_ov must be 2-D (naux, nocc*nvir)")
+        if b.shape[1] != nocc * nvir:
+            raise LayoutError(f"b_ov is {b.shape}, expected width {nocc * nvir}")
+        st = L.fh_rimp2_pair_energies(b.ctypes.data_as(dp), b.shape[0], nocc, nvir,
+                                      e.ctypes.data_as(dp), e.size, first_occ, nocc_total,
+                                      int(precision), e_os.ctypes.data_as(dp),
+                                      e_ss.ctypes.data_as(dp))
+    check(st, "fh_rimp2_pair_energies")
+    return PairEnergies(nocc, e_os + e_ss, e_os, e_ss)
+
+
+def build_pair_domains_by_energy(centers, pair_energies: PairEnergies, t_cut_pairs: float,
+                                 coupling_cutoff_bohr: float = math.inf) -> PairDomains:
+    """pair_energy_screen.rs:206-282: pairs i <= j with |e_ij| >= t_cut_pairs,
+    diagonals always kept; coupled[p] = the pairs whose centroids lie within
+    the coupling cutoff of p's (p itself always)."""
+    c = np.asarray(centers, dtype=np.float64)
+    if c.ndim != 2 or c.shape[1] != 3:
+        raise LayoutError(f"Boys centers must be (nocc, 3); got {c.shape}")
+    if c.shape[0] != pair_energies.nocc:
+        raise LayoutError(f"centers has {c.shape[0]} rows but pair energies were built for "
+                          f"nocc={pair_energies.nocc}")
+    if not t_cut_pairs >= 0.0:
+        raise LayoutError(f"t_cut_pairs must be >= 0 (got {t_cut_pairs}); use 0.0 to disable")
+    if coupling_cutoff_bohr < 0.0:
+        raise LayoutError(f"coupling cutoff must be >= 0 (got {coupling_cutoff_bohr})")
+    nocc = c.shape[0]
+    pairs = [(i, j) for i in range(nocc) for j in range(i, nocc)
+             if i == j or abs(pair_energies.e_ij(i, j)) >= t_cut_pairs]
+    cents = np.array([0.5 * (c[i] + c[j]) for i, j in pairs]).reshape(len(pairs), 3)
+    cut_sq = coupling_cutoff_bohr * coupling_cutoff_bohr
+    coupled = []
+    for p in range(len(pairs)):
+        d = np.sum((cents - cents[p]) ** 2, axis=1)
+        coupled.append([q for q in range(len(pairs)) if q == p or d[q] <= cut_sq])
+    return PairDomains(nocc, pairs, coupled, c.copy(), math.inf, coupling_cutoff_bohr)
+
+
+def dlpno_mp2_from_b_ov(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int,
+                        domains: PairDomains, cfg: DlpnoConfig = DlpnoConfig(),
+                        precision: Precision = Precision.FP64,
+                        pair_energies: PairEnergies | None = None):
+    """dlpno_mp2.rs:670-760 for t_cut_pno == 0: (SpinComponents, DlpnoDiagnostics)
+    summed over the retained pairs of ``domains`` in their order. Pass the
+    ``pair_energies`` already computed for the screen to reuse them."""
+    if cfg.t_cut_pno < 0.0:
+        raise LayoutError(f"t_cut_pno must be >= 0 (got {cfg.t_cut_pno}); use 0.0 to disable "
+                          "truncation")
+    if cfg.t_cut_pno > 0.0:
+        raise UnsupportedError("PNO truncation (t_cut_pno > 0) is not on the device")
+    if domains.nocc != nocc:
+        raise LayoutError(f"domains were built for nocc={domains.nocc}, but nocc={nocc}")
+    pe = pair_energies
+    if pe is None:
+        pe = estimate_pair_energies_from_b_ov(b_ov, eps, nocc, nvir, first_occ, nocc_total,
+                                              precision)
+    e_os = math.fsum(pe.e_os[i, j] for i, j in domains.pairs)
+    e_ss = math.fsum(pe.e_ss[i, j] for i, j in domains.pairs)
+    diag = DlpnoDiagnostics(domains.pair_retention(), 1.0, 0.0, len(domains.pairs))
+    return SpinComponents(e_os, e_ss, e_os + e_ss), diag

@@ -92,6 +92,31 @@ def mixed_large(seed: int = 1003, nocc: int = 100, nvir: int = 900, naux: int = 
                    nvir, naux)
 
 
+def local_chain(seed: int = 1006, nocc: int = 24, nvir: int = 72, rows_per_site: int = 24,
+                reach: int = 3, spacing: float = 2.5):
+    """(Problem, centers): localized occupied orbitals on a line (x = spacing * i
+    bohr) whose B columns live on the aux rows of the sites within ``reach`` of
+    their own, with weight exp(-|k - i|). Pairs further apart than 2 * reach
+    have exactly zero pair energy; the nearer ones decay with distance, so an
+    energy screen keeps a band of pairs. Occupied U[-1.2, -0.4], virtuals
+    log-uniform on [0.2, 4.0]."""
+    rng = np.random.default_rng(seed)
+    naux = nocc * rows_per_site
+    b = np.zeros((naux, nocc * nvir))
+    for i in range(nocc):
+        for k in range(max(0, i - reach), min(nocc, i + reach + 1)):
+            blk = rng.standard_normal((rows_per_site, nvir)) * math.exp(-abs(k - i))
+            b[k * rows_per_site:(k + 1) * rows_per_site, i * nvir:(i + 1) * nvir] = blk
+    b *= math.sqrt(0.0081 / rows_per_site)
+    occ = np.sort(rng.uniform(-1.2, -0.4, nocc))
+    vir = np.sort(_loguniform(rng, 0.2, 4.0, nvir))
+    centers = np.zeros((nocc, 3))
+    centers[:, 0] = spacing * np.arange(nocc)
+    prob = Problem(f"local chain {nocc}/{nvir}/{naux}", seed, b, np.concatenate([occ, vir]),
+                   nocc, nvir, naux)
+    return prob, centers
+
+
 @dataclass
 class DomainSet:
     """Independent local-MP2 pair domains packed into flat buffers."""

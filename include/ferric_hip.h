@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 4
+#define FERRIC_HIP_ABI_VERSION 5
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -240,6 +240,27 @@ fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom, const doub
                                size_t b_len, const double *eps_dev, size_t eps_len,
                                fh_precision precision, int use_graph, void *stream,
                                double *e_os, double *e_ss);
+
+/* ---- Pair energies (the local-MP2 pair screen) --------------------------- */
+
+/* estimate_pair_energies_from_b_ov (crates/ferric-mp2/src/dlpno_mp2.rs:527-594):
+ * the MP2 energy of every occupied pair from one pass of the fused kernel, as
+ * its opposite-spin part e_os[i][j] = fac sum_ab (ia|jb)^2 / D and same-spin
+ * part e_ss[i][j] = fac sum_ab (ia|jb)((ia|jb) - (ib|ja)) / D (fac = 1 for
+ * i == j, 2 otherwise; the reference's estimate e_ij is their sum). e_os and
+ * e_ss are host nocc x nocc row-major, symmetric. A non-negative denominator
+ * (2 max eps_occ >= 2 min eps_vir) is FH_LAYOUT, as the reference's refusal;
+ * so is a NaN among the active orbital energies.
+ * Any precision of fh_rimp2_energy. The ABI version is 5 since these entries
+ * were added. */
+fh_status fh_rimp2_pair_energies(const double *b_ov, size_t naux, size_t nocc, size_t nvir,
+                                 const double *eps, size_t n_eps, size_t first_occ,
+                                 size_t nocc_total, fh_precision precision, double *e_os,
+                                 double *e_ss);
+fh_status fh_rimp2_pair_energies_dev(const void *b_dev, fh_dtype b_dtype, size_t naux,
+                                     size_t nocc, size_t nvir, const double *eps, size_t n_eps,
+                                     size_t first_occ, size_t nocc_total, fh_precision precision,
+                                     void *stream, double *e_os, double *e_ss);
 
 /* ---- Dressing / MO transform upstream of the pair stage ---------------- */
 

@@ -315,3 +315,31 @@ def dress_b_ov(j3, v_inv_sqrt, c_left, c_right, output_band=None):
     if rc:
         raise OracleError(f"oracle error {rc}")
     return out
+
+
+def pair_energies(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None):
+    """estimate_pair_energies_from_b_ov (crates/ferric-mp2/src/dlpno_mp2.rs:527-594)
+    split into its parts: (e_os, e_ss), each nocc x nocc symmetric with
+    e_os[i, j] = fac sum_ab g_ab^2 / D and e_ss[i, j] = fac sum_ab g_ab (g_ab - g_ba) / D,
+    g = B_i^T B_j, fac = 1 (i == j) or 2; their sum is the reference's e_ij. A
+    non-negative denominator raises ValueError, as the reference refuses it.
+    numpy, one nvir x nvir block per pair."""
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    eo = e[first_occ:first_occ + nocc]
+    ev = e[nocc_total:nocc_total + nvir]
+    e_os = np.zeros((nocc, nocc))
+    e_ss = np.zeros((nocc, nocc))
+    for i in range(nocc):
+        bi = b[:, i * nvir:(i + 1) * nvir]
+        for j in range(i, nocc):
+            g = bi.T @ b[:, j * nvir:(j + 1) * nvir]
+            d = eo[i] + eo[j] - ev[:, None] - ev[None, :]
+            if np.any(d >= 0.0):
+                raise ValueError(f"non-negative MP2 denominator for pair ({i}, {j})")
+            fac = 1.0 if i == j else 2.0
+            e_os[i, j] = e_os[j, i] = fac * float(np.sum(g * g / d))
+            e_ss[i, j] = e_ss[j, i] = fac * float(np.sum(g * (g - g.T) / d))
+    return e_os, e_ss
