@@ -71,6 +71,10 @@ class DomainC(ctypes.Structure):
                 ("naux", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
+class PnoDiagnosticsC(ctypes.Structure):
+    _fields_ = [("kept_pnos", ctypes.c_uint64), ("max_discarded_weight", ctypes.c_double)]
+
+
 class SpinChannelC(ctypes.Structure):
     _fields_ = [("b", ctypes.c_void_p), ("nocc", ctypes.c_size_t), ("nvir", ctypes.c_size_t),
                 ("eps", ctypes.POINTER(ctypes.c_double)), ("n_eps", ctypes.c_size_t),
@@ -144,6 +148,12 @@ SIGNATURES = {
                                         _c_int, _c_dp, _c_dp]),
     "fh_rimp2_pair_energies_dev": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz,
                                             _c_sz, _c_sz, _c_int, _c_vp, _c_dp, _c_dp]),
+    "fh_dlpno_pno_pair_energies": (_c_int, [_c_dp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz,
+                                            _c_sz, _c_vp, _c_sz, ctypes.c_double, _c_dp, _c_dp,
+                                            ctypes.POINTER(PnoDiagnosticsC)]),
+    "fh_dlpno_pno_pair_energies_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz,
+                                                _c_sz, _c_vp, _c_sz, ctypes.c_double, _c_vp,
+                                                _c_dp, _c_dp, ctypes.POINTER(PnoDiagnosticsC)]),
     "fh_dress_b_ov_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz,
                                    _c_vp, _c_vp]),
     "fh_u_same_spin": (_c_int, [ctypes.POINTER(SpinChannelC), _c_sz, _c_int, _c_dp]),
@@ -162,7 +172,7 @@ SIGNATURES = {
 }
 
 _lib = None
-ABI_VERSION = 5   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 6   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

@@ -39,6 +39,8 @@ enum SlotId {
     kSlotTSplit,     // INT8X4 adjoint: BF16 hi and lo of the whole symmetric T~
     kSlotG32,        // INT8X4 adjoint: FP32 X = B T~ of the rocBLAS GEMMs
     kSlotPairs,      // pair energies: (e_os, e_ss) per unique pair
+    kSlotPnoWork,    // PNO: five nvir x nvir matrices and three vectors per batched pair
+    kSlotPnoMeta,    // PNO: pair list, per-pair results, masks, GEMM pointer arrays
     kNumSlots
 };
 

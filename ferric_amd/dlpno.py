@@ -10,14 +10,18 @@ crates/ferric-mp2/src/pair_energy_screen.rs:
   :func:`build_pair_domains_by_energy` (pair_energy_screen.rs:206-282): pairs
   ``i <= j`` kept when ``|e_ij| >= t_cut_pairs`` (diagonals always), and the
   geometric coupling lists on pair centroids;
-* :func:`dlpno_mp2_from_b_ov` (dlpno_mp2.rs:670-760) for ``t_cut_pno == 0``.
-  There every pair's PNO transform is a square orthogonal rotation, and the
+* :func:`pno_pair_energies` (``fh_dlpno_pno_pair_energies[_dev]``): the
+  per-pair body of dlpno_mp2_from_b_ov with PNO truncation -- pair density,
+  its eigenvectors above ``t_cut_pno`` (local_pno.rs:132-200), semicanonical
+  rotation and the energy in that basis (dlpno_mp2.rs:249-325), batched over
+  pairs on the device;
+* :func:`dlpno_mp2_from_b_ov` (dlpno_mp2.rs:670-760). With ``t_cut_pno == 0``
+  every pair's PNO transform is a square orthogonal rotation, and the
   semicanonical rotation after it re-diagonalises the virtual Fock block, so
   both cancel out of the pair energy (the reference's exactness contract,
-  dlpno_mp2.rs:797-830): the DLPNO energy is the sum of the retained pairs'
-  canonical energies, which the device computes. PNO truncation
-  (``t_cut_pno > 0``) is not on the device and raises UnsupportedError, for the
-  caller's CPU path.
+  dlpno_mp2.rs:797-830): the energy is the sum of the retained pairs' canonical
+  energies, which the fused pair kernel computes without any eigensolve. With
+  ``t_cut_pno > 0`` the retained pairs go through :func:`pno_pair_energies`.
 """
 from __future__ import annotations
 
@@ -27,7 +31,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import LayoutError, Precision, UnsupportedError, check, lib
+from ._lib import LayoutError, PnoDiagnosticsC, Precision, UnsupportedError, check, lib
 from .rimp2 import SpinComponents, _is_torch_cuda
 
 DEFAULT_T_CUT_PAIRS = 1e-5  # dlpno_mp2.rs:121
@@ -180,20 +184,72 @@ def build_pair_domains_by_energy(centers, pair_energies: PairEnergies, t_cut_pai
     return PairDomains(nocc, pairs, coupled, c.copy(), math.inf, coupling_cutoff_bohr)
 
 
+def pno_pair_energies(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int, pairs,
+                      t_cut_pno: float):
+    """Per-pair energies in each pair's truncated, semicanonical PNO basis
+    (dlpno_mp2.rs:700-740 per pair): (e_os, e_ss, kept_pnos, max_discarded_weight)
+    with e_os, e_ss arrays over ``pairs`` (mirror factor applied), FP64.
+    ``b_ov``: numpy array (host entry) or contiguous float64 CUDA tensor (device
+    entry on the current torch stream)."""
+    pl = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    npairs = pl.shape[0]
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    e_os = np.zeros(npairs)
+    e_ss = np.zeros(npairs)
+    diag = PnoDiagnosticsC()
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    if _is_torch_cuda(b_ov):
+        import torch
+
+        if b_ov.dim() != 2 or not b_ov.is_contiguous() or b_ov.dtype != torch.float64:
+            raise LayoutError("b_ov must be a contiguous 2-D float64 (naux, nocc*nvir) tensor")
+        if b_ov.shape[1] != nocc * nvir:
+            raise LayoutError(f"b_ov is {tuple(b_ov.shape)}, expected width {nocc * nvir}")
+        stream = torch.cuda.current_stream(b_ov.device).cuda_stream
+        st = L.fh_dlpno_pno_pair_energies_dev(
+            ctypes.c_void_p(b_ov.data_ptr()), b_ov.shape[0], nocc, nvir, e.ctypes.data_as(dp),
+            e.size, first_occ, nocc_total, ctypes.c_void_p(pl.ctypes.data), npairs,
+            float(t_cut_pno), ctypes.c_void_p(stream), e_os.ctypes.data_as(dp),
+            e_ss.ctypes.data_as(dp), ctypes.byref(diag))
+    else:
+        b = np.ascontiguousarray(np.asarray(b_ov, dtype=np.float64))
+        if b.ndim != 2 or b.shape[1] != nocc * nvir:
+            raise LayoutError(f"b_ov is {b.shape}, expected (naux, {nocc * nvir})")
+        st = L.fh_dlpno_pno_pair_energies(
+            b.ctypes.data_as(dp), b.shape[0], nocc, nvir, e.ctypes.data_as(dp), e.size,
+            first_occ, nocc_total, ctypes.c_void_p(pl.ctypes.data), npairs, float(t_cut_pno),
+            e_os.ctypes.data_as(dp), e_ss.ctypes.data_as(dp), ctypes.byref(diag))
+    check(st, "fh_dlpno_pno_pair_energies")
+    return e_os, e_ss, int(diag.kept_pnos), float(diag.max_discarded_weight)
+
+
 def dlpno_mp2_from_b_ov(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int,
                         domains: PairDomains, cfg: DlpnoConfig = DlpnoConfig(),
                         precision: Precision = Precision.FP64,
                         pair_energies: PairEnergies | None = None):
-    """dlpno_mp2.rs:670-760 for t_cut_pno == 0: (SpinComponents, DlpnoDiagnostics)
-    summed over the retained pairs of ``domains`` in their order. Pass the
-    ``pair_energies`` already computed for the screen to reuse them."""
+    """dlpno_mp2.rs:670-760: (SpinComponents, DlpnoDiagnostics) summed over the
+    retained pairs of ``domains`` in their order. At ``t_cut_pno == 0`` the
+    canonical pair energies serve (pass the ``pair_energies`` already computed
+    for the screen to reuse them); above it the PNO path runs, in FP64."""
     if cfg.t_cut_pno < 0.0:
         raise LayoutError(f"t_cut_pno must be >= 0 (got {cfg.t_cut_pno}); use 0.0 to disable "
                           "truncation")
-    if cfg.t_cut_pno > 0.0:
-        raise UnsupportedError("PNO truncation (t_cut_pno > 0) is not on the device")
     if domains.nocc != nocc:
         raise LayoutError(f"domains were built for nocc={domains.nocc}, but nocc={nocc}")
+    if not cfg.t_cut_pno == 0.0:
+        if precision != Precision.FP64:
+            raise UnsupportedError("PNO truncation runs in FP64")
+        pos, pss, kept, worst = pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total,
+                                                  domains.pairs, cfg.t_cut_pno)
+        e_os = e_ss = 0.0
+        for k in range(len(domains.pairs)):  # the reference's order of accumulation
+            e_os += pos[k]
+            e_ss += pss[k]
+        n = len(domains.pairs)
+        vret = 1.0 if n == 0 or nvir == 0 else kept / (n * nvir)
+        return (SpinComponents(e_os, e_ss, e_os + e_ss),
+                DlpnoDiagnostics(domains.pair_retention(), vret, worst, n))
     pe = pair_energies
     if pe is None:
         pe = estimate_pair_energies_from_b_ov(b_ov, eps, nocc, nvir, first_occ, nocc_total,

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 5
+#define FERRIC_HIP_ABI_VERSION 6
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -261,6 +261,37 @@ fh_status fh_rimp2_pair_energies_dev(const void *b_dev, fh_dtype b_dtype, size_t
                                      size_t nocc, size_t nvir, const double *eps, size_t n_eps,
                                      size_t first_occ, size_t nocc_total, fh_precision precision,
                                      void *stream, double *e_os, double *e_ss);
+
+/* PNO truncation (dlpno_mp2.rs:670-760 with t_cut_pno >= 0): for each listed
+ * pair, pairs[2k] = i <= pairs[2k+1] = j < nocc (active indices), the pair
+ * density D = sym(T T^T + T^T T) of the amplitudes T = g / D_ij
+ * (local_pno.rs:132-200), its eigenvectors with |occupation| >= t_cut_pno
+ * (never none: the largest is kept), the semicanonical rotation of those PNOs
+ * and the pair energy in that basis (dlpno_mp2.rs:249-325): e_os[k], e_ss[k]
+ * (host, npairs, the mirror factor applied). diag (may be null) receives the
+ * total kept PNO count and the largest discarded weight, from which
+ * virtual_retention = kept_pnos / (npairs * nvir) (dlpno_mp2.rs:464-477).
+ * FP64 only (b_dev is FP64). t_cut_pno < 0, a malformed pair, or a
+ * non-negative denominator in the listed pairs is FH_LAYOUT; an eigensolver
+ * failure is FH_KERNEL. At t_cut_pno = 0 every pair energy equals the
+ * canonical one of fh_rimp2_pair_energies to rounding. The ABI version is 6
+ * since these entries were added. */
+typedef struct fh_pno_diagnostics {
+    uint64_t kept_pnos;
+    double max_discarded_weight;
+} fh_pno_diagnostics;
+
+fh_status fh_dlpno_pno_pair_energies(const double *b_ov, size_t naux, size_t nocc, size_t nvir,
+                                     const double *eps, size_t n_eps, size_t first_occ,
+                                     size_t nocc_total, const int32_t *pairs, size_t npairs,
+                                     double t_cut_pno, double *e_os, double *e_ss,
+                                     fh_pno_diagnostics *diag);
+fh_status fh_dlpno_pno_pair_energies_dev(const double *b_dev, size_t naux, size_t nocc,
+                                         size_t nvir, const double *eps, size_t n_eps,
+                                         size_t first_occ, size_t nocc_total,
+                                         const int32_t *pairs, size_t npairs, double t_cut_pno,
+                                         void *stream, double *e_os, double *e_ss,
+                                         fh_pno_diagnostics *diag);
 
 /* ---- Dressing / MO transform upstream of the pair stage ---------------- */
 

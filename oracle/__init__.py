@@ -343,3 +343,43 @@ def pair_energies(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None):
             e_os[i, j] = e_os[j, i] = fac * float(np.sum(g * g / d))
             e_ss[i, j] = e_ss[j, i] = fac * float(np.sum(g * (g - g.T) / d))
     return e_os, e_ss
+
+
+def pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total, pairs, t_cut_pno):
+    """The pair loop of dlpno_mp2_from_b_ov (crates/ferric-mp2/src/dlpno_mp2.rs:718-744)
+    per pair: amplitudes T = g / D (:700-712), pno_for_pair (local_pno.rs:132-200:
+    D = sym(T T^T + T^T T), eigh, keep |n| >= t_cut_pno in descending order, never
+    none), pair_energy_in_pno_basis (dlpno_mp2.rs:249-340: Q^T g Q, the PNO Fock
+    block re-diagonalised, the energy with the semicanonical denominators).
+    Returns (e_os, e_ss, kept, discarded, kept_occupation) arrays over pairs
+    (the last, the sum of the kept occupations, is for the trace identity of
+    the tests); numpy eigh."""
+    b = _f64(b_ov)
+    e = _f64(eps)
+    pairs = [tuple(int(x) for x in p) for p in pairs]
+    n = len(pairs)
+    e_os, e_ss = np.zeros(n), np.zeros(n)
+    kept, disc, kocc = np.zeros(n, dtype=np.int64), np.zeros(n), np.zeros(n)
+    ev = e[nocc_total:nocc_total + nvir]
+    for k, (i, j) in enumerate(pairs):
+        g = b[:, i * nvir:(i + 1) * nvir].T @ b[:, j * nvir:(j + 1) * nvir]
+        eij = e[first_occ + i] + e[first_occ + j]
+        t = g / (eij - ev[:, None] - ev[None, :])
+        d = t @ t.T + t.T @ t
+        d = 0.5 * (d + d.T)
+        occ, vec = np.linalg.eigh(d)
+        order = sorted(range(nvir), key=lambda x: -occ[x])  # stable, descending
+        keep = [x for x in order if abs(occ[x]) >= t_cut_pno] or [order[0]]
+        disc[k] = sum(abs(occ[x]) for x in order if x not in keep)
+        kept[k] = len(keep)
+        kocc[k] = sum(occ[x] for x in keep)
+        q = vec[:, keep]
+        tij, tji = q.T @ g @ q, q.T @ g.T @ q
+        f = q.T @ (ev[:, None] * q)
+        e_pno, u = np.linalg.eigh(f)
+        tij, tji = u.T @ tij @ u, u.T @ tji @ u
+        dd = eij - e_pno[:, None] - e_pno[None, :]
+        fac = 1.0 if i == j else 2.0
+        e_os[k] = fac * float(np.sum(tij * tij / dd))
+        e_ss[k] = fac * float(np.sum(tij * (tij - tji) / dd))
+    return e_os, e_ss, kept, disc, kocc

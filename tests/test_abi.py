@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.fh_abi_version() == 5
+    assert L.fh_abi_version() == 6
 
 
 def test_enum_values_match_header():
@@ -48,6 +48,28 @@ def test_struct_layouts():
     assert ctypes.sizeof(_lib.StatsC) == 18 * 8
     assert ctypes.sizeof(_lib.DomainC) == 32
     assert ctypes.sizeof(_lib.TimingC) == 48
+    assert ctypes.sizeof(_lib.PnoDiagnosticsC) == 16
+
+
+def test_pno_entry_checks_before_the_device():
+    """t_cut_pno < 0 (local_pno.rs:112-118) and malformed pairs are FH_LAYOUT
+    before any device work, so they are testable without a GPU."""
+    b = np.zeros((4, 6))
+    eps = np.array([-1.0, -0.8, 0.5, 0.6, 0.7])
+    L = _lib.lib()
+
+    def call(pairs, t):
+        pl = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        os_, ss = np.zeros(len(pl)), np.zeros(len(pl))
+        return L.fh_dlpno_pno_pair_energies(
+            b.ctypes.data_as(_lib._c_dp), 4, 2, 3, eps.ctypes.data_as(_lib._c_dp), 5, 0, 2,
+            ctypes.c_void_p(pl.ctypes.data), len(pl), t, os_.ctypes.data_as(_lib._c_dp),
+            ss.ctypes.data_as(_lib._c_dp), None)
+
+    assert call([(0, 1)], -1e-3) == _lib.Status.LAYOUT
+    assert call([(1, 0)], 0.0) == _lib.Status.LAYOUT
+    assert call([(0, 2)], 0.0) == _lib.Status.LAYOUT
+    assert call([(-1, 1)], 0.0) == _lib.Status.LAYOUT
 
 
 def _energy(b, naux, nocc, nvir, eps, first_occ, nocc_total, kappa=float("nan"), prec=0):

@@ -12,8 +12,15 @@
   exact configuration reproduces the dense energy; on a chain of localized
   orbitals the default 1e-5 Eh screen drops the far pairs and the energy
   moves by exactly the discarded pairs' energy (pair_energy_screen.rs:90-105).
-* Refusals: a non-negative denominator (pair_energy_screen.rs:494-502) and
-  PNO truncation, which is not on the device.
+* PNO truncation (fh_dlpno_pno_pair_energies): at t_cut_pno = 0 equal to the
+  canonical pairs (the exactness contract, dlpno_mp2.rs:797-830); above it,
+  per pair against the oracle's restatement at 1e-10 of the largest pair, with
+  the same kept PNO counts and discarded weights, on a local chain over four
+  thresholds, the reference's own toy system and frozen-core golden cases;
+  a huge threshold keeps one PNO per pair; batching and the host entry change
+  nothing; the driver with the default screen plus truncation.
+* Refusals: a non-negative denominator (pair_energy_screen.rs:494-502),
+  FP32 input to the PNO path, reduced precision with truncation.
 """
 import math
 import os
@@ -27,7 +34,8 @@ torch = pytest.importorskip("torch")
 
 from ferric_amd import LayoutError, Precision, UnsupportedError, rimp2, synthetic  # noqa: E402
 from ferric_amd.dlpno import (DlpnoConfig, build_pair_domains_by_energy,  # noqa: E402
-                              dlpno_mp2_from_b_ov, estimate_pair_energies_from_b_ov)
+                              dlpno_mp2_from_b_ov, estimate_pair_energies_from_b_ov,
+                              pno_pair_energies)
 from tests import quant  # noqa: E402
 
 FIX = os.path.join(os.path.dirname(__file__), "golden", "water_ccpvdz_real.npz")
@@ -165,8 +173,108 @@ def test_refusals():
     with pytest.raises(LayoutError):
         estimate_pair_energies_from_b_ov(_cuda(b), eps, 3, 5)
     p, centers = synthetic.local_chain(nocc=4, nvir=8, rows_per_site=4)
+    with pytest.raises(LayoutError):  # (ia|jb) with a zero gap in the listed pair
+        pno_pair_energies(b, eps, 3, 4, 0, 3, [(2, 2)], 1e-3)
+    with pytest.raises(LayoutError):
+        pno_pair_energies(_cuda(p.b_ov).float(), p.eps, p.nocc, p.nvir, 0, p.nocc, [(0, 0)], 0.0)
     pe = estimate_pair_energies_from_b_ov(p.b_ov, p.eps, p.nocc, p.nvir)
     d = build_pair_domains_by_energy(centers, pe, 0.0)
     with pytest.raises(UnsupportedError):
         dlpno_mp2_from_b_ov(p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, d,
-                            DlpnoConfig(t_cut_pno=1e-7))
+                            DlpnoConfig(t_cut_pno=1e-7), precision=Precision.FP32)
+
+
+# -- PNO truncation (dlpno_mp2.rs:249-352, :700-744; local_pno.rs:132-200) ------
+
+def _pairs(nocc):
+    return [(i, j) for i in range(nocc) for j in range(i, nocc)]
+
+
+def _pno_check(oracle_mod, b, eps, nocc, nvir, fo, nt, pairs, t_cut, rel=1e-10):
+    w_os, w_ss, w_kept, w_disc, _ = oracle_mod.pno_pair_energies(b, eps, nocc, nvir, fo, nt,
+                                                                 pairs, t_cut)
+    g_os, g_ss, kept, worst = pno_pair_energies(_cuda(b), eps, nocc, nvir, fo, nt, pairs, t_cut)
+    scale = np.max(np.abs(w_os))
+    print(f"  t_cut {t_cut:g}: kept {kept} of {len(pairs) * nvir}, max |dE_pair| "
+          f"{np.max(np.abs(g_os - w_os)):.2e} {np.max(np.abs(g_ss - w_ss)):.2e}", flush=True)
+    assert kept == int(w_kept.sum())
+    assert abs(worst - np.max(w_disc)) <= 1e-10 * max(np.max(w_disc), 1e-300)
+    assert np.max(np.abs(g_os - w_os)) <= rel * scale
+    assert np.max(np.abs(g_ss - w_ss)) <= rel * scale
+    return g_os, g_ss, kept, worst
+
+
+def test_pno_zero_truncation_equals_the_canonical_pairs():
+    p = synthetic.decane_scale()
+    print("pno t_cut 0 vs canonical pairs, decane-scale", flush=True)
+    t = _cuda(p.b_ov)
+    pe = estimate_pair_energies_from_b_ov(t, p.eps, p.nocc, p.nvir)
+    pairs = [(i, j) for i in range(0, p.nocc, 4) for j in range(i, p.nocc, 5)]
+    g_os, g_ss, kept, worst = pno_pair_energies(t, p.eps, p.nocc, p.nvir, 0, p.nocc, pairs, 0.0)
+    want_os = np.array([pe.e_os[i, j] for i, j in pairs])
+    want_ss = np.array([pe.e_ss[i, j] for i, j in pairs])
+    scale = np.max(np.abs(want_os))
+    assert kept == len(pairs) * p.nvir and worst == 0.0
+    assert np.max(np.abs(g_os - want_os)) <= 1e-11 * scale
+    assert np.max(np.abs(g_ss - want_ss)) <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("t_cut", [1e-8, 1e-6, 1e-4, 1e-2])
+def test_pno_truncation_against_oracle_local_chain(oracle_mod, t_cut):
+    p, _ = synthetic.local_chain(nocc=10, nvir=40, rows_per_site=12)
+    print("pno local chain 10/40", flush=True)
+    _pno_check(oracle_mod, p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, _pairs(p.nocc), t_cut)
+
+
+def test_pno_reference_toy_and_frozen_core(oracle_mod, golden):
+    from tests.test_oracle_pno import toy_b_ov
+
+    b, eps = toy_b_ov(14, 4, 6)
+    print("pno reference toy 14/4/6", flush=True)
+    _pno_check(oracle_mod, b, eps, 4, 6, 0, 4, _pairs(4), 1e-2)
+    for name, c in golden.items():
+        if c["first_occ"] == 0:
+            continue
+        print("pno frozen core", name, flush=True)
+        nocc, nvir = c["nocc"], c["nvir"]
+        _pno_check(oracle_mod, c["b"], c["eps"], nocc, nvir, c["first_occ"], c["nocc_total"],
+                   _pairs(nocc), 1e-5)
+
+
+def test_pno_never_empties_a_pair(oracle_mod):
+    p, _ = synthetic.local_chain(nocc=5, nvir=24, rows_per_site=8)
+    print("pno huge threshold", flush=True)
+    g_os, g_ss, kept, _ = _pno_check(oracle_mod, p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc,
+                                     _pairs(p.nocc), 1e300)
+    assert kept == len(_pairs(p.nocc))
+    assert np.all(np.abs(g_ss) <= 1e-12 * np.abs(g_os))
+
+
+def test_pno_batches_host_entry_and_driver(oracle_mod, monkeypatch):
+    p, centers = synthetic.local_chain(nocc=8, nvir=32, rows_per_site=10)
+    print("pno batches / host entry / driver", flush=True)
+    pairs = _pairs(p.nocc)
+    one = pno_pair_energies(_cuda(p.b_ov), p.eps, p.nocc, p.nvir, 0, p.nocc, pairs, 1e-5)
+    host = pno_pair_energies(p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, pairs, 1e-5)
+    assert np.array_equal(one[0], host[0]) and np.array_equal(one[1], host[1])
+    assert one[2:] == host[2:]
+    monkeypatch.setenv("FERRIC_PNO_BATCH", "5")
+    few = pno_pair_energies(_cuda(p.b_ov), p.eps, p.nocc, p.nvir, 0, p.nocc, pairs, 1e-5)
+    assert few[2] == one[2]
+    scale = np.max(np.abs(one[0]))
+    assert np.max(np.abs(few[0] - one[0])) <= 1e-12 * scale
+    assert np.max(np.abs(few[1] - one[1])) <= 1e-12 * scale
+    monkeypatch.delenv("FERRIC_PNO_BATCH")
+    # the driver: default pair screen plus PNO truncation, against the oracle's sum
+    pe = estimate_pair_energies_from_b_ov(p.b_ov, p.eps, p.nocc, p.nvir)
+    cfg = DlpnoConfig(t_cut_pno=1e-6)
+    d = build_pair_domains_by_energy(centers, pe, cfg.t_cut_pairs)
+    sc, diag = dlpno_mp2_from_b_ov(_cuda(p.b_ov), p.eps, p.nocc, p.nvir, 0, p.nocc, d, cfg)
+    w_os, w_ss, w_kept, w_disc, _ = oracle_mod.pno_pair_energies(p.b_ov, p.eps, p.nocc, p.nvir,
+                                                                 0, p.nocc, d.pairs, 1e-6)
+    assert abs(sc.e_os - np.sum(w_os)) <= 1e-10 * abs(sc.e_total)
+    assert abs(sc.e_ss - np.sum(w_ss)) <= 1e-10 * abs(sc.e_total)
+    assert diag.virtual_retention == w_kept.sum() / (len(d.pairs) * p.nvir) < 1.0
+    assert diag.n_pairs == len(d.pairs) and diag.max_discarded_weight > 0.0
+    dense = rimp2.spin_components(_cuda(p.b_ov), p.eps, p.nocc, p.nvir)
+    assert sc.e_total > dense.e_total  # truncation loses correlation

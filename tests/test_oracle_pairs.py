@@ -15,7 +15,7 @@ import os
 import numpy as np
 import pytest
 
-from ferric_amd import LayoutError, UnsupportedError
+from ferric_amd import LayoutError, Precision, UnsupportedError
 from ferric_amd.dlpno import (DlpnoConfig, PairDomains, PairEnergies,
                               build_pair_domains_by_energy, dlpno_mp2_from_b_ov)
 
@@ -174,8 +174,9 @@ def test_dlpno_config_checks_before_any_device_call(oracle_mod):
     eps = np.array([-1.0, -0.8, -0.6, 0.4, 0.7, 1.1, 1.6])
     with pytest.raises(LayoutError):
         dlpno_mp2_from_b_ov(b, eps, 3, 4, 0, 3, d, DlpnoConfig(t_cut_pno=-1.0), pair_energies=pe)
-    with pytest.raises(UnsupportedError):
-        dlpno_mp2_from_b_ov(b, eps, 3, 4, 0, 3, d, DlpnoConfig(t_cut_pno=1e-8), pair_energies=pe)
+    with pytest.raises(UnsupportedError):  # PNO truncation runs in FP64
+        dlpno_mp2_from_b_ov(b, eps, 3, 4, 0, 3, d, DlpnoConfig(t_cut_pno=1e-8),
+                            precision=Precision.BF16X3, pair_energies=pe)
     with pytest.raises(LayoutError):
         dlpno_mp2_from_b_ov(b, eps, 4, 3, 0, 4, d, pair_energies=pe)
     assert DlpnoConfig.exact().is_exact() and not DlpnoConfig().is_exact()
