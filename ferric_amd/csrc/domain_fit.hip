@@ -1,0 +1,284 @@
+// Domain-local RI fits per occupied pair on the device: domain_fit_pair
+// (crates/ferric-mp2/src/lmp2_amplitude.rs:1707-1758),
+//
+//   J_ij = A_{i,D}^T V_DD^{-1} A_{j,D},   D = the aux functions within `radius`
+//                                          of the centre of i or of j,
+//
+// with A the unwhitened (P|ia) and V the aux metric. With V_DD = L L^T this is
+// (L^-1 A_{i,D})^T (L^-1 A_{j,D}). Pairs are batched at one padded domain size
+// dmax: a pair's V_DD sits in the leading d x d block with the identity on the
+// rest of the diagonal, and its A rows beyond d are zero, so the padding leaves
+// L^-1 A and J unchanged. The Cholesky factorisations run on rocSOLVER
+// (dpotrf), the triangular solves and the products on rocBLAS; the gathers are
+// the kernels below.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <mutex>
+#include <vector>
+
+#include "fh_host.h"
+#include "fh_kernels.h"
+
+namespace fh {
+namespace {
+
+constexpr int kThreads = 256;
+
+fh_status solver_ok(rocblas_status st, const char *what) {
+    if (st == rocblas_status_success) return FH_OK;
+    return fail(FH_KERNEL, "%s: rocBLAS/rocSOLVER status %d", what, (int)st);
+}
+
+// One workgroup per pair. vp (dmax x dmax, column-major): V[dom[r], dom[c]]
+// inside the domain, the identity on the padded diagonal, zero elsewhere.
+// x (dmax x 2 nvir, column-major): columns a < nvir hold A[dom[r], i nvir + a],
+// columns nvir + b hold A[dom[r], j nvir + b]; zero on padded rows.
+__global__ __launch_bounds__(kThreads) void fit_gather_kernel(
+    const int32_t *__restrict__ pairs, const int32_t *__restrict__ dom,
+    const int32_t *__restrict__ dsize, int dmax, const double *__restrict__ v, int64_t naux,
+    const double *__restrict__ a, int64_t nov, int nvir, double *__restrict__ vp,
+    double *__restrict__ x) {
+    const int p = blockIdx.x;
+    const int32_t *dp = dom + (int64_t)p * dmax;
+    const int d = dsize[p];
+    const int i = pairs[2 * p], j = pairs[2 * p + 1];
+    double *vq = vp + (int64_t)p * dmax * dmax;
+    for (int64_t e = threadIdx.x; e < (int64_t)dmax * dmax; e += kThreads) {
+        const int r = (int)(e % dmax), c = (int)(e / dmax);
+        double val;
+        if (r < d && c < d)
+            val = v[(int64_t)dp[r] * naux + dp[c]];
+        else
+            val = r == c ? 1.0 : 0.0;
+        vq[e] = val;
+    }
+    double *xq = x + (int64_t)p * dmax * 2 * nvir;
+    for (int64_t e = threadIdx.x; e < (int64_t)dmax * 2 * nvir; e += kThreads) {
+        const int r = (int)(e % dmax), col = (int)(e / dmax);
+        double val = 0.0;
+        if (r < d) {
+            const int occ = col < nvir ? i : j;
+            const int av = col < nvir ? col : col - nvir;
+            val = a[(int64_t)dp[r] * nov + (int64_t)occ * nvir + av];
+        }
+        xq[e] = val;
+    }
+}
+
+size_t fit_batch(size_t npairs, size_t dmax, size_t nvir) {
+    size_t nb = 0;
+    if (const char *s = std::getenv("FERRIC_FIT_BATCH")) nb = (size_t)std::strtoull(s, nullptr, 10);
+    if (nb == 0) {
+        const size_t per = sizeof(double) * (dmax * dmax + 2 * dmax * nvir) + 64;
+        nb = std::max<size_t>(1, ((size_t)4 << 30) / per);
+    }
+    return std::min(nb, npairs);
+}
+
+fh_status fit_core(State &s, const double *a, const double *v, size_t naux, size_t nocc,
+                   size_t nvir, const double *aux_xyz, const double *centers, double radius,
+                   const int32_t *pairs, size_t npairs, hipStream_t stream, double *j_out,
+                   size_t *max_domain) {
+    // Domains on the host, in aux order (lmp2_amplitude.rs:1726-1735).
+    const double r2 = radius * radius;
+    std::vector<int32_t> sizes(npairs);
+    std::vector<std::vector<int32_t>> doms(npairs);
+    size_t dmax = 0;
+    for (size_t k = 0; k < npairs; ++k) {
+        const int32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+        for (size_t p = 0; p < naux; ++p) {
+            double di = 0.0, dj = 0.0;
+            for (int x = 0; x < 3; ++x) {
+                const double ui = aux_xyz[3 * p + x] - centers[3 * (size_t)i + x];
+                const double uj = aux_xyz[3 * p + x] - centers[3 * (size_t)j + x];
+                di += ui * ui;
+                dj += uj * uj;
+            }
+            if (di <= r2 || dj <= r2) doms[k].push_back((int32_t)p);
+        }
+        if (doms[k].empty())
+            return fail(FH_LAYOUT, "empty aux domain for pair (%d, %d) at radius %g bohr", i, j,
+                        radius);
+        sizes[k] = (int32_t)doms[k].size();
+        dmax = std::max(dmax, doms[k].size());
+    }
+    if (max_domain) *max_domain = dmax;
+    std::vector<int32_t> flat(npairs * dmax, 0);
+    for (size_t k = 0; k < npairs; ++k)
+        std::copy(doms[k].begin(), doms[k].end(), flat.begin() + k * dmax);
+
+    const size_t nb = fit_batch(npairs, dmax, nvir);
+    const size_t vmat = dmax * dmax, xmat = dmax * 2 * nvir, jmat = nvir * nvir;
+    fh_status st = reserve(s, {{kSlotFitWork, sizeof(double) * nb * (vmat + xmat)},
+                               {kSlotFitMeta, sizeof(int32_t) * (npairs * (dmax + 3) + nb)}});
+    if (st != FH_OK) return st;
+    double *Vp = (double *)s.slots[kSlotFitWork].ptr;
+    double *X = Vp + nb * vmat;
+    int32_t *ddom = (int32_t *)s.slots[kSlotFitMeta].ptr;
+    int32_t *dsz = ddom + npairs * dmax;
+    int32_t *dpairs = dsz + npairs;
+    int32_t *info = dpairs + 2 * npairs;
+    st = hip_check(hipMemcpyAsync(ddom, flat.data(), sizeof(int32_t) * flat.size(),
+                                  hipMemcpyHostToDevice, stream),
+                   "H2D domains");
+    if (st != FH_OK) return st;
+    st = hip_check(hipMemcpyAsync(dsz, sizes.data(), sizeof(int32_t) * npairs,
+                                  hipMemcpyHostToDevice, stream),
+                   "H2D domain sizes");
+    if (st != FH_OK) return st;
+    st = hip_check(hipMemcpyAsync(dpairs, pairs, sizeof(int32_t) * 2 * npairs,
+                                  hipMemcpyHostToDevice, stream),
+                   "H2D pairs");
+    if (st != FH_OK) return st;
+    s.pending.h2d_bytes += sizeof(int32_t) * (flat.size() + 3 * npairs);
+
+    void *hv = nullptr;
+    st = blas_handle(s, stream, &hv);
+    if (st != FH_OK) return st;
+    rocblas_handle h = (rocblas_handle)hv;
+    const double one = 1.0, zero = 0.0;
+    const int D = (int)dmax, V = (int)nvir;
+    std::vector<int32_t> hinfo(nb);
+    mark(s, 0, stream);
+    mark(s, 1, stream);
+    for (size_t p0 = 0; p0 < npairs; p0 += nb) {
+        const int n = (int)std::min(nb, npairs - p0);
+        hipLaunchKernelGGL(fit_gather_kernel, dim3(n), dim3(kThreads), 0, stream,
+                           dpairs + 2 * p0, ddom + p0 * dmax, dsz + p0, D, v, (int64_t)naux, a,
+                           (int64_t)(nocc * nvir), V, Vp, X);
+        st = hip_check(hipGetLastError(), "domain gather");
+        if (st != FH_OK) return st;
+        st = solver_ok(rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, D, Vp, D,
+                                                        (rocblas_stride)vmat, info, n),
+                       "domain metric Cholesky");
+        if (st != FH_OK) return st;
+        // X <- L^-1 X
+        st = solver_ok(rocblas_dtrsm_strided_batched(
+                           h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, D, 2 * V, &one, Vp, D,
+                           (rocblas_stride)vmat, X, D, (rocblas_stride)xmat, n),
+                       "domain triangular solve");
+        if (st != FH_OK) return st;
+        // Row-major J[a][b] = sum_r X[r, a] X[r, V + b]: column-major J^T = Xj^T Xi.
+        st = solver_ok(rocblas_dgemm_strided_batched(
+                           h, rocblas_operation_transpose, rocblas_operation_none, V, V, D, &one,
+                           X + (size_t)D * V, D, (rocblas_stride)xmat, X, D,
+                           (rocblas_stride)xmat, &zero, j_out + p0 * jmat, V,
+                           (rocblas_stride)jmat, n),
+                       "domain fit GEMM");
+        if (st != FH_OK) return st;
+        s.pending.kernel_launches += 4;
+        st = hip_check(hipMemcpyAsync(hinfo.data(), info, sizeof(int32_t) * n,
+                                      hipMemcpyDeviceToHost, stream),
+                       "D2H Cholesky info");
+        if (st != FH_OK) return st;
+        st = hip_check(hipStreamSynchronize(stream), "stream sync");
+        if (st != FH_OK) return st;
+        for (int k = 0; k < n; ++k)
+            if (hinfo[k] != 0)
+                return fail(FH_KERNEL,
+                            "domain metric of pair (%d, %d) is not positive definite (Cholesky "
+                            "info %d)",
+                            pairs[2 * (p0 + k)], pairs[2 * (p0 + k) + 1], hinfo[k]);
+    }
+    mark(s, 2, stream);
+    mark(s, 3, stream);
+    finish_timing(s, (uint64_t)npairs, 0);
+    return FH_OK;
+}
+
+fh_status fit_checks(size_t naux, size_t nocc, size_t nvir, const double *aux_xyz,
+                     const double *centers, double radius, const int32_t *pairs, size_t npairs) {
+    if (naux == 0 || nocc == 0 || nvir == 0) return fail(FH_LAYOUT, "empty dimension");
+    if (!aux_xyz || !centers) return fail(FH_LAYOUT, "null aux positions or occupied centres");
+    if (!(radius >= 0.0)) return fail(FH_LAYOUT, "radius must be >= 0 (got %g)", radius);
+    if (npairs > 0 && !pairs) return fail(FH_LAYOUT, "null pair list");
+    for (size_t k = 0; k < npairs; ++k) {
+        const int32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+        if (i < 0 || i > j || (size_t)j >= nocc)
+            return fail(FH_LAYOUT, "pair %zu is (%d, %d): need 0 <= i <= j < nocc = %zu", k, i,
+                        j, nocc);
+    }
+    if (naux > 46340 || nocc * nvir > (size_t)INT32_MAX)
+        return fail(FH_LAYOUT, "naux %zu / nocc*nvir %zu beyond the int32 BLAS range", naux,
+                    nocc * nvir);
+    return FH_OK;
+}
+
+}  // namespace
+}  // namespace fh
+
+using namespace fh;
+
+extern "C" fh_status fh_domain_fit_pair_blocks_dev(const double *a_dev, const double *v_dev,
+                                                   size_t naux, size_t nocc, size_t nvir,
+                                                   const double *aux_xyz, const double *centers,
+                                                   double radius, const int32_t *pairs,
+                                                   size_t npairs, void *stream, double *j_dev,
+                                                   size_t *max_domain) {
+    State &s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    clear_error();
+    s.stats.calls++;
+    s.pending = Pending{};
+    fh_status st = fit_checks(naux, nocc, nvir, aux_xyz, centers, radius, pairs, npairs);
+    if (st != FH_OK) return st;
+    if (max_domain) *max_domain = 0;
+    if (npairs == 0) return FH_OK;
+    if (!a_dev || !v_dev || !j_dev) return fail(FH_LAYOUT, "null device pointer");
+    st = ensure_device(s);
+    if (st != FH_OK) return st;
+    st = fit_core(s, a_dev, v_dev, naux, nocc, nvir, aux_xyz, centers, radius, pairs, npairs,
+                  (hipStream_t)stream, j_dev, max_domain);
+    if (st != FH_OK) return st;
+    commit(s, Prec::F64, false, false);
+    return FH_OK;
+}
+
+extern "C" fh_status fh_domain_fit_pair_blocks(const double *a, const double *v, size_t naux,
+                                               size_t nocc, size_t nvir, const double *aux_xyz,
+                                               const double *centers, double radius,
+                                               const int32_t *pairs, size_t npairs, double *j_out,
+                                               size_t *max_domain) {
+    State &s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    clear_error();
+    s.stats.calls++;
+    s.pending = Pending{};
+    fh_status st = fit_checks(naux, nocc, nvir, aux_xyz, centers, radius, pairs, npairs);
+    if (st != FH_OK) return st;
+    if (max_domain) *max_domain = 0;
+    if (npairs == 0) return FH_OK;
+    if (!a || !v || !j_out) return fail(FH_LAYOUT, "null pointer");
+    st = ensure_device(s);
+    if (st != FH_OK) return st;
+    const size_t abytes = sizeof(double) * naux * nocc * nvir;
+    const size_t vbytes = sizeof(double) * naux * naux;
+    const size_t jbytes = sizeof(double) * npairs * nvir * nvir;
+    st = reserve(s, {{kSlotStage, abytes}, {kSlotStage2, vbytes}, {kSlotGamma, jbytes}});
+    if (st != FH_OK) return st;
+    st = hip_check(hipMemcpyAsync(s.slots[kSlotStage].ptr, a, abytes, hipMemcpyHostToDevice,
+                                  nullptr),
+                   "H2D A");
+    if (st != FH_OK) return st;
+    st = hip_check(hipMemcpyAsync(s.slots[kSlotStage2].ptr, v, vbytes, hipMemcpyHostToDevice,
+                                  nullptr),
+                   "H2D V");
+    if (st != FH_OK) return st;
+    s.pending.h2d_bytes += abytes + vbytes;
+    double *jd = (double *)s.slots[kSlotGamma].ptr;
+    st = fit_core(s, (const double *)s.slots[kSlotStage].ptr,
+                  (const double *)s.slots[kSlotStage2].ptr, naux, nocc, nvir, aux_xyz, centers,
+                  radius, pairs, npairs, nullptr, jd, max_domain);
+    if (st != FH_OK) return st;
+    st = hip_check(hipMemcpy(j_out, jd, jbytes, hipMemcpyDeviceToHost), "D2H J");
+    if (st != FH_OK) return st;
+    s.pending.d2h_bytes += jbytes;
+    commit(s, Prec::F64, false, false);
+    return FH_OK;
+}

@@ -41,6 +41,8 @@ enum SlotId {
     kSlotPairs,      // pair energies: (e_os, e_ss) per unique pair
     kSlotPnoWork,    // PNO: five nvir x nvir matrices and three vectors per batched pair
     kSlotPnoMeta,    // PNO: pair list, per-pair results, masks, GEMM pointer arrays
+    kSlotFitWork,    // domain fits: padded V_DD and [A_i | A_j] per batched pair
+    kSlotFitMeta,    // domain fits: domain index lists, sizes, pairs, Cholesky info
     kNumSlots
 };
 

@@ -15,6 +15,9 @@ crates/ferric-mp2/src/pair_energy_screen.rs:
   its eigenvectors above ``t_cut_pno`` (local_pno.rs:132-200), semicanonical
   rotation and the energy in that basis (dlpno_mp2.rs:249-325), batched over
   pairs on the device;
+* :func:`domain_fit_pair_blocks` (``fh_domain_fit_pair_blocks[_dev]``): the
+  domain-local RI fit J_ij = A_{i,D}^T V_DD^-1 A_{j,D} of each pair over the aux
+  functions near either of its centres (lmp2_amplitude.rs:1707-1758);
 * :func:`dlpno_mp2_from_b_ov` (dlpno_mp2.rs:670-760). With ``t_cut_pno == 0``
   every pair's PNO transform is a square orthogonal rotation, and the
   semicanonical rotation after it re-diagonalises the virtual Fock block, so
@@ -222,6 +225,55 @@ def pno_pair_energies(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_tota
             e_os.ctypes.data_as(dp), e_ss.ctypes.data_as(dp), ctypes.byref(diag))
     check(st, "fh_dlpno_pno_pair_energies")
     return e_os, e_ss, int(diag.kept_pnos), float(diag.max_discarded_weight)
+
+
+def domain_fit_pair_blocks(a, v, aux_xyz, occ_centers, nocc: int, nvir: int, radius: float,
+                           pairs=None):
+    """lmp2_amplitude.rs:1707-1758 for every listed pair (all i <= j when
+    ``pairs`` is None): (J, max_domain) with J[k] the nvir x nvir block of pair
+    k, J[k][a, b] = (ia|jb) fitted in the pair's aux domain. ``a`` (naux,
+    nocc*nvir) unwhitened (P|ia) and ``v`` (naux, naux): both numpy (host entry,
+    J numpy) or both contiguous float64 CUDA tensors (device entry on the
+    current torch stream, J a CUDA tensor)."""
+    if pairs is None:
+        pairs = [(i, j) for i in range(nocc) for j in range(i, nocc)]
+    pl = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    npairs = pl.shape[0]
+    xyz = np.ascontiguousarray(np.asarray(aux_xyz, dtype=np.float64))
+    cen = np.ascontiguousarray(np.asarray(occ_centers, dtype=np.float64))
+    if xyz.ndim != 2 or xyz.shape[1] != 3 or cen.shape != (nocc, 3):
+        raise LayoutError(f"aux_xyz must be (naux, 3) and centers ({nocc}, 3); got {xyz.shape}, "
+                          f"{cen.shape}")
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    dmax = ctypes.c_size_t()
+    if _is_torch_cuda(a):
+        import torch
+
+        for t, shape in ((a, (xyz.shape[0], nocc * nvir)), (v, (xyz.shape[0], xyz.shape[0]))):
+            if (not _is_torch_cuda(t) or t.dtype != torch.float64 or not t.is_contiguous()
+                    or tuple(t.shape) != shape):
+                raise LayoutError(f"expected a contiguous float64 CUDA tensor of shape {shape}")
+        out = torch.empty((npairs, nvir, nvir), dtype=torch.float64, device=a.device)
+        stream = torch.cuda.current_stream(a.device).cuda_stream
+        st = L.fh_domain_fit_pair_blocks_dev(
+            ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(v.data_ptr()), xyz.shape[0], nocc,
+            nvir, xyz.ctypes.data_as(dp), cen.ctypes.data_as(dp), float(radius),
+            ctypes.c_void_p(pl.ctypes.data), npairs, ctypes.c_void_p(stream),
+            ctypes.c_void_p(out.data_ptr()), ctypes.byref(dmax))
+    else:
+        an = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        vn = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+        if an.shape != (xyz.shape[0], nocc * nvir) or vn.shape != (xyz.shape[0], xyz.shape[0]):
+            raise LayoutError(f"a {an.shape} / v {vn.shape} do not match naux={xyz.shape[0]}, "
+                              f"nocc*nvir={nocc * nvir}")
+        out = np.empty((npairs, nvir, nvir))
+        st = L.fh_domain_fit_pair_blocks(
+            an.ctypes.data_as(dp), vn.ctypes.data_as(dp), xyz.shape[0], nocc, nvir,
+            xyz.ctypes.data_as(dp), cen.ctypes.data_as(dp), float(radius),
+            ctypes.c_void_p(pl.ctypes.data), npairs, out.ctypes.data_as(dp), ctypes.byref(dmax))
+    check(st, "fh_domain_fit_pair_blocks")
+    return out, int(dmax.value)
 
 
 def dlpno_mp2_from_b_ov(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int,

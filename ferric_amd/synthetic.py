@@ -117,6 +117,30 @@ def local_chain(seed: int = 1006, nocc: int = 24, nvir: int = 72, rows_per_site:
     return prob, centers
 
 
+def local_fit_chain(seed: int = 1007, nocc: int = 8, nvir: int = 16, aux_per_site: int = 6,
+                    spacing: float = 2.5):
+    """(a, v, aux_xyz, centers, eps) for domain-local fits: occupied centres on a
+    line (x = spacing * i bohr), aux functions spread around each site, an SPD
+    metric v[P, Q] = exp(-(x_P - x_Q)^2 / 4) + 0.5 delta_PQ, and an unwhitened
+    (P|ia) whose rows decay as exp(-|x_P - x_i| / spacing) away from centre i."""
+    rng = np.random.default_rng(seed)
+    naux = nocc * aux_per_site
+    xs = np.repeat(spacing * np.arange(nocc), aux_per_site) + rng.uniform(-0.8, 0.8, naux)
+    aux_xyz = np.zeros((naux, 3))
+    aux_xyz[:, 0] = xs
+    aux_xyz[:, 1] = rng.uniform(-0.3, 0.3, naux)
+    centers = np.zeros((nocc, 3))
+    centers[:, 0] = spacing * np.arange(nocc)
+    dx = xs[:, None] - xs[None, :]
+    v = np.exp(-dx * dx / 4.0) + 0.5 * np.eye(naux)
+    a = rng.standard_normal((naux, nocc * nvir)) * 0.1
+    for i in range(nocc):
+        a[:, i * nvir:(i + 1) * nvir] *= np.exp(-np.abs(xs - centers[i, 0]) / spacing)[:, None]
+    occ = np.sort(rng.uniform(-1.2, -0.4, nocc))
+    vir = np.sort(_loguniform(rng, 0.2, 4.0, nvir))
+    return a, v, aux_xyz, centers, np.concatenate([occ, vir])
+
+
 @dataclass
 class DomainSet:
     """Independent local-MP2 pair domains packed into flat buffers."""

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 6
+#define FERRIC_HIP_ABI_VERSION 7
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -292,6 +292,27 @@ fh_status fh_dlpno_pno_pair_energies_dev(const double *b_dev, size_t naux, size_
                                          const int32_t *pairs, size_t npairs, double t_cut_pno,
                                          void *stream, double *e_os, double *e_ss,
                                          fh_pno_diagnostics *diag);
+
+/* Domain-local RI fit blocks (domain_fit_pair, lmp2_amplitude.rs:1707-1758):
+ * for each listed pair (pairs[2k] = i <= pairs[2k+1] = j < nocc),
+ *   J_ij[a][b] = sum_{P,Q in D} A[P][i nvir + a] (V_DD^-1)[P][Q] A[Q][j nvir + b],
+ * D = the aux functions P with |aux_xyz[P] - centers[i]| <= radius or
+ * |aux_xyz[P] - centers[j]| <= radius (bohr). a: the unwhitened (P|ia),
+ * (naux, nocc*nvir) row-major; v: the aux metric (naux, naux); aux_xyz
+ * (naux, 3) and centers (nocc, 3) are host arrays. j_out receives npairs
+ * row-major nvir x nvir blocks (device for _dev, host otherwise); max_domain
+ * (may be null) the largest |D|. An empty domain or a malformed pair is
+ * FH_LAYOUT; a V_DD that is not positive definite is FH_KERNEL. FP64. The ABI
+ * version is 7 since these entries were added. */
+fh_status fh_domain_fit_pair_blocks(const double *a, const double *v, size_t naux, size_t nocc,
+                                    size_t nvir, const double *aux_xyz, const double *centers,
+                                    double radius, const int32_t *pairs, size_t npairs,
+                                    double *j_out, size_t *max_domain);
+fh_status fh_domain_fit_pair_blocks_dev(const double *a_dev, const double *v_dev, size_t naux,
+                                        size_t nocc, size_t nvir, const double *aux_xyz,
+                                        const double *centers, double radius,
+                                        const int32_t *pairs, size_t npairs, void *stream,
+                                        double *j_dev, size_t *max_domain);
 
 /* ---- Dressing / MO transform upstream of the pair stage ---------------- */
 

@@ -383,3 +383,28 @@ def pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total, pairs, t_cut
         e_os[k] = fac * float(np.sum(tij * tij / dd))
         e_ss[k] = fac * float(np.sum(tij * (tij - tji) / dd))
     return e_os, e_ss, kept, disc, kocc
+
+
+def domain_fit_pair(a, v, aux_xyz, occ_centers, i, j, nvir, radius):
+    """domain_fit_pair (crates/ferric-mp2/src/lmp2_amplitude.rs:1707-1758): the
+    aux domain D = {P : |x_P - c_i|^2 <= r^2 or |x_P - c_j|^2 <= r^2} in aux
+    order, V_DD inverted through its Cholesky factor (ndarray-linalg invc), then
+    A_{i,D}^T (V_DD^-1 A_{j,D}). Raises ValueError on an empty domain."""
+    import scipy.linalg
+
+    a = _f64(a)
+    v = _f64(v)
+    x = _f64(aux_xyz)
+    c = _f64(occ_centers)
+    r2 = radius * radius
+    d2i = np.sum((x - c[i]) ** 2, axis=1)
+    d2j = np.sum((x - c[j]) ** 2, axis=1)
+    dom = [p for p in range(x.shape[0]) if d2i[p] <= r2 or d2j[p] <= r2]
+    if not dom:
+        raise ValueError(f"empty aux domain for pair ({i},{j}) at radius {radius} bohr")
+    vdd = v[np.ix_(dom, dom)]
+    cf = scipy.linalg.cho_factor(vdd, lower=True)
+    vinv = scipy.linalg.cho_solve(cf, np.eye(len(dom)))
+    a_i = a[dom, i * nvir:(i + 1) * nvir]
+    a_j = a[dom, j * nvir:(j + 1) * nvir]
+    return a_i.T @ (vinv @ a_j), len(dom)

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.fh_abi_version() == 6
+    assert L.fh_abi_version() == 7
 
 
 def test_enum_values_match_header():
@@ -70,6 +70,25 @@ def test_pno_entry_checks_before_the_device():
     assert call([(1, 0)], 0.0) == _lib.Status.LAYOUT
     assert call([(0, 2)], 0.0) == _lib.Status.LAYOUT
     assert call([(-1, 1)], 0.0) == _lib.Status.LAYOUT
+
+
+def test_domain_fit_entry_checks_before_the_device():
+    a, v = np.zeros((4, 6)), np.eye(4)
+    xyz, cen = np.zeros((4, 3)), np.zeros((2, 3))
+    L = _lib.lib()
+
+    def call(pairs, radius):
+        pl = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        out = np.zeros((len(pl), 3, 3))
+        return L.fh_domain_fit_pair_blocks(
+            a.ctypes.data_as(_lib._c_dp), v.ctypes.data_as(_lib._c_dp), 4, 2, 3,
+            xyz.ctypes.data_as(_lib._c_dp), cen.ctypes.data_as(_lib._c_dp), radius,
+            ctypes.c_void_p(pl.ctypes.data), len(pl), out.ctypes.data_as(_lib._c_dp), None)
+
+    assert call([(0, 1)], -1.0) == _lib.Status.LAYOUT
+    assert call([(0, 1)], float("nan")) == _lib.Status.LAYOUT
+    assert call([(1, 0)], 1.0) == _lib.Status.LAYOUT
+    assert call([(0, 2)], 1.0) == _lib.Status.LAYOUT
 
 
 def _energy(b, naux, nocc, nvir, eps, first_occ, nocc_total, kappa=float("nan"), prec=0):

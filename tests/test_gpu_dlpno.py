@@ -19,8 +19,13 @@
   thresholds, the reference's own toy system and frozen-core golden cases;
   a huge threshold keeps one PNO per pair; batching and the host entry change
   nothing; the driver with the default screen plus truncation.
+* Domain-local fits (fh_domain_fit_pair_blocks): every pair's block against
+  the oracle's restatement at 1e-12 at three radii (uneven domains in one
+  batch), host and device entries equal, batching; an unbounded radius gives
+  the canonical pair energies.
 * Refusals: a non-negative denominator (pair_energy_screen.rs:494-502),
-  FP32 input to the PNO path, reduced precision with truncation.
+  FP32 input to the PNO path, reduced precision with truncation, an empty
+  aux domain, a metric that is not positive definite.
 """
 import math
 import os
@@ -32,10 +37,11 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-from ferric_amd import LayoutError, Precision, UnsupportedError, rimp2, synthetic  # noqa: E402
+from ferric_amd import (KernelError, LayoutError, Precision, UnsupportedError, rimp2,  # noqa: E402
+                        synthetic)
 from ferric_amd.dlpno import (DlpnoConfig, build_pair_domains_by_energy,  # noqa: E402
-                              dlpno_mp2_from_b_ov, estimate_pair_energies_from_b_ov,
-                              pno_pair_energies)
+                              dlpno_mp2_from_b_ov, domain_fit_pair_blocks,
+                              estimate_pair_energies_from_b_ov, pno_pair_energies)
 from tests import quant  # noqa: E402
 
 FIX = os.path.join(os.path.dirname(__file__), "golden", "water_ccpvdz_real.npz")
@@ -278,3 +284,61 @@ def test_pno_batches_host_entry_and_driver(oracle_mod, monkeypatch):
     assert diag.n_pairs == len(d.pairs) and diag.max_discarded_weight > 0.0
     dense = rimp2.spin_components(_cuda(p.b_ov), p.eps, p.nocc, p.nvir)
     assert sc.e_total > dense.e_total  # truncation loses correlation
+
+
+# -- domain-local fits (lmp2_amplitude.rs:1707-1758) --------------------------------
+
+def test_domain_fit_blocks_against_oracle(oracle_mod, monkeypatch):
+    a, v, xyz, cen, eps = synthetic.local_fit_chain(nocc=8, nvir=16, aux_per_site=6)
+    nocc, nvir = cen.shape[0], 16
+    pairs = _pairs(nocc)
+    for radius in (3.0, 6.0, math.inf):
+        print(f"domain fit radius {radius}", flush=True)
+        jd, dmax = domain_fit_pair_blocks(_cuda(a), _cuda(v), xyz, cen, nocc, nvir, radius, pairs)
+        jd = jd.cpu().numpy()
+        sizes = []
+        for k, (i, j) in enumerate(pairs):
+            want, d = oracle_mod.domain_fit_pair(a, v, xyz, cen, i, j, nvir, radius)
+            sizes.append(d)
+            assert np.max(np.abs(jd[k] - want)) <= 1e-12 * np.max(np.abs(want)), (radius, i, j)
+        assert dmax == max(sizes)
+        if math.isinf(radius):
+            assert min(sizes) == a.shape[0]
+        else:
+            assert min(sizes) < dmax  # uneven domains in one batch
+    jh, _ = domain_fit_pair_blocks(a, v, xyz, cen, nocc, nvir, 3.0, pairs)
+    jd, _ = domain_fit_pair_blocks(_cuda(a), _cuda(v), xyz, cen, nocc, nvir, 3.0, pairs)
+    assert np.array_equal(jh, jd.cpu().numpy())
+    monkeypatch.setenv("FERRIC_FIT_BATCH", "7")
+    jb, _ = domain_fit_pair_blocks(_cuda(a), _cuda(v), xyz, cen, nocc, nvir, 3.0, pairs)
+    assert np.max(np.abs(jb.cpu().numpy() - jh)) <= 1e-13 * np.max(np.abs(jh))
+
+
+def test_domain_fit_unbounded_gives_the_canonical_pairs(oracle_mod):
+    a, v, xyz, cen, eps = synthetic.local_fit_chain(nocc=6, nvir=12, aux_per_site=5)
+    nocc, nvir = 6, 12
+    print("domain fit unbounded vs canonical pair energies", flush=True)
+    jd, _ = domain_fit_pair_blocks(_cuda(a), _cuda(v), xyz, cen, nocc, nvir, math.inf)
+    b = np.linalg.solve(np.linalg.cholesky(v), a)
+    pe = estimate_pair_energies_from_b_ov(_cuda(b), eps, nocc, nvir)
+    ev = eps[nocc:]
+    for k, (i, j) in enumerate(_pairs(nocc)):
+        g = jd[k].cpu().numpy()
+        d = eps[i] + eps[j] - ev[:, None] - ev[None, :]
+        fac = 1.0 if i == j else 2.0
+        e = fac * np.sum((2 * g - g.T) * g / d)
+        assert abs(e - pe.e[i, j]) <= 1e-11 * np.max(np.abs(pe.e))
+
+
+def test_domain_fit_refusals():
+    a, v, xyz, cen, _ = synthetic.local_fit_chain(nocc=3, nvir=4, aux_per_site=4)
+    print("domain fit refusals", flush=True)
+    far = cen + np.array([0.0, 100.0, 0.0])
+    with pytest.raises(LayoutError):
+        domain_fit_pair_blocks(_cuda(a), _cuda(v), xyz, far, 3, 4, 1.0)
+    bad = v.copy()
+    bad[0, 0] = -1.0  # not positive definite in any domain holding aux 0
+    with pytest.raises(KernelError):
+        domain_fit_pair_blocks(_cuda(a), _cuda(bad), xyz, cen, 3, 4, math.inf, [(0, 1)])
+    with pytest.raises(LayoutError):
+        domain_fit_pair_blocks(_cuda(a).float(), _cuda(v), xyz, cen, 3, 4, 1.0)
