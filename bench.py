@@ -421,6 +421,64 @@ def main():
         log(f"signed decane: {med:.2f} ms")
         del pos, neg, ca, cb
 
+        # The local-MP2 pair layer on a chain of 100 localized occupied orbitals
+        # (200 virtuals, 2400 aux rows, 5050 pairs): per-pair energies (one fused
+        # pass), the default DLPNO configuration (pair screen at 1e-5 Eh, no PNO
+        # truncation), PNO truncation at 1e-6 over the retained pairs, and
+        # domain-local fits on a 40-site chain (480 aux functions, radius 6 bohr).
+        from ferric_amd import dlpno
+
+        ch, cen = synthetic.local_chain(nocc=100, nvir=200, rows_per_site=24)
+        cb_ = torch.from_numpy(ch.b_ov).to(dev)
+        loc = {"system": ch.name}
+        dense = rimp2.spin_components(cb_, ch.eps, ch.nocc, ch.nvir)
+        pe = dlpno.estimate_pair_energies_from_b_ov(cb_, ch.eps, ch.nocc, ch.nvir)
+        med, mn = timed_ms(lambda: dlpno.estimate_pair_energies_from_b_ov(cb_, ch.eps, ch.nocc,
+                                                                          ch.nvir), 10)
+        loc["pair_energies_fp64"] = {"ms_median": med, "ms_min": mn,
+                                     "sum_minus_E_total": pe.total() - dense.e_total}
+        cfg = dlpno.DlpnoConfig()
+
+        def default_dlpno():
+            pe2 = dlpno.estimate_pair_energies_from_b_ov(cb_, ch.eps, ch.nocc, ch.nvir)
+            dom = dlpno.build_pair_domains_by_energy(cen, pe2, cfg.t_cut_pairs)
+            return dlpno.dlpno_mp2_from_b_ov(cb_, ch.eps, ch.nocc, ch.nvir, 0, ch.nocc, dom, cfg,
+                                             pair_energies=pe2), dom
+
+        (sc_d, diag_d), dom = default_dlpno()
+        med, mn = timed_ms(default_dlpno, 5)
+        loc["dlpno_default"] = {"ms_median": med, "ms_min": mn, "E_total": sc_d.e_total,
+                                "dE_vs_dense": sc_d.e_total - dense.e_total,
+                                "pairs_kept": len(dom.pairs),
+                                "pair_retention": diag_d.pair_retention}
+        cfg6 = dlpno.DlpnoConfig(t_cut_pno=1e-6)
+        sc_p, diag_p = dlpno.dlpno_mp2_from_b_ov(cb_, ch.eps, ch.nocc, ch.nvir, 0, ch.nocc, dom,
+                                                 cfg6)
+        med, mn = timed_ms(lambda: dlpno.dlpno_mp2_from_b_ov(cb_, ch.eps, ch.nocc, ch.nvir, 0,
+                                                             ch.nocc, dom, cfg6), 3)
+        loc["pno_1e-6"] = {"ms_median": med, "ms_min": mn, "E_total": sc_p.e_total,
+                           "dE_vs_untruncated": sc_p.e_total - sc_d.e_total,
+                           "virtual_retention": diag_p.virtual_retention,
+                           "max_discarded_weight": diag_p.max_discarded_weight,
+                           "pairs": len(dom.pairs), "nvir": ch.nvir}
+        a_f, v_f, xyz_f, cen_f, _ = synthetic.local_fit_chain(nocc=40, nvir=100,
+                                                              aux_per_site=12)
+        at, vt = torch.from_numpy(a_f).to(dev), torch.from_numpy(v_f).to(dev)
+        _, dmax = dlpno.domain_fit_pair_blocks(at, vt, xyz_f, cen_f, 40, 100, 6.0)
+        med, mn = timed_ms(lambda: dlpno.domain_fit_pair_blocks(at, vt, xyz_f, cen_f, 40, 100,
+                                                                6.0), 3)
+        loc["domain_fit_40x100_480aux_r6"] = {"ms_median": med, "ms_min": mn, "pairs": 820,
+                                              "max_domain": dmax}
+        small["local MP2"] = loc
+        log(f"local MP2 ({ch.name}): pair energies {loc['pair_energies_fp64']['ms_median']:.2f} "
+            f"ms, default DLPNO {loc['dlpno_default']['ms_median']:.2f} ms ({len(dom.pairs)} "
+            f"pairs, dE {loc['dlpno_default']['dE_vs_dense']:+.2e}), PNO 1e-6 "
+            f"{loc['pno_1e-6']['ms_median']:.1f} ms (retention "
+            f"{loc['pno_1e-6']['virtual_retention']:.3f}, dE "
+            f"{loc['pno_1e-6']['dE_vs_untruncated']:+.2e}), domain fits "
+            f"{loc['domain_fit_40x100_480aux_r6']['ms_median']:.1f} ms")
+        del cb_, at, vt
+
     ok = [v for v in variants if v != Precision.FP64 and abs(res[v.name]["dE_vs_fp64"]) <= 1e-6]
     head = min(ok, key=lambda v: res[v.name]["ms_per_step"]) if ok else Precision.FP64
     h = res[head.name]

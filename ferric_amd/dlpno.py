@@ -176,14 +176,27 @@ def build_pair_domains_by_energy(centers, pair_energies: PairEnergies, t_cut_pai
     if coupling_cutoff_bohr < 0.0:
         raise LayoutError(f"coupling cutoff must be >= 0 (got {coupling_cutoff_bohr})")
     nocc = c.shape[0]
-    pairs = [(i, j) for i in range(nocc) for j in range(i, nocc)
-             if i == j or abs(pair_energies.e_ij(i, j)) >= t_cut_pairs]
-    cents = np.array([0.5 * (c[i] + c[j]) for i, j in pairs]).reshape(len(pairs), 3)
+    iu, ju = np.triu_indices(nocc)  # row-major i <= j, the reference's loop order
+    e = np.asarray(pair_energies.e, dtype=np.float64)
+    keep = (iu == ju) | (np.abs(e[iu, ju]) >= t_cut_pairs)
+    pi, pj = iu[keep], ju[keep]
+    pairs = list(zip(pi.tolist(), pj.tolist()))
+    cents = 0.5 * (c[pi] + c[pj])
     cut_sq = coupling_cutoff_bohr * coupling_cutoff_bohr
-    coupled = []
-    for p in range(len(pairs)):
-        d = np.sum((cents - cents[p]) ** 2, axis=1)
-        coupled.append([q for q in range(len(pairs)) if q == p or d[q] <= cut_sq])
+    n = len(pairs)
+    if cut_sq == math.inf:  # every centroid distance is within an infinite cutoff
+        everything = list(range(n))
+        coupled = [everything.copy() for _ in range(n)]
+    else:
+        coupled = []
+        for r0 in range(0, n, 256):
+            blk = cents[r0:r0 + 256]
+            d = np.zeros((blk.shape[0], n))
+            for ax in range(3):  # the reference's axis order of the sum
+                d += (blk[:, ax:ax + 1] - cents[None, :, ax]) ** 2
+            m = d <= cut_sq
+            m[np.arange(blk.shape[0]), np.arange(r0, r0 + blk.shape[0])] = True
+            coupled.extend(np.flatnonzero(row).tolist() for row in m)
     return PairDomains(nocc, pairs, coupled, c.copy(), math.inf, coupling_cutoff_bohr)
 
 
