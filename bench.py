@@ -144,6 +144,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-dressing", action="store_true")
+    ap.add_argument("--no-from-g", action="store_true")
     ap.add_argument("--no-small-configs", action="store_true",
                     help="skip the water-shape and decane-scale legs")
     ap.add_argument("--cpu-rows", default="50:100", help="occupied rows of the CPU sample")
@@ -321,6 +322,60 @@ def main():
         log(f"dressing {dressing['shape']}: {ms:.1f} ms, {dressing['tflops']:.1f} TFLOP/s FP64")
         del j3, dm, cl, cr, out_b
         torch.cuda.empty_cache()
+
+    # spin_components_from_g at the headline shape: the dense (ia|jb) matrix
+    # g = B^T B (nov^2 FP64 = 64.8 GB, built by one GEMM outside the timing),
+    # read once over its i <= j blocks -- an HBM-bound pass (algorithmic bytes
+    # = nocc(nocc+1)/2 nvir^2 8).
+    from_g = {}
+    if not args.no_from_g:
+        nov = p.nocc * p.nvir
+        free = torch.cuda.mem_get_info(dev)[0]
+        if free > 1.3 * 8 * nov * nov:
+            b64 = b.double()
+            g = b64.t() @ b64
+            del b64
+            torch.cuda.synchronize()
+            sc_g = rimp2.spin_components_from_g(g, p.eps, p.nocc, p.nvir)
+            ks = []
+            for _ in range(max(3, args.steps)):
+                rimp2.spin_components_from_g(g, p.eps, p.nocc, p.nvir)
+                ks.append(last_timing()["contract_ms"])
+            kms = float(np.median(ks))
+            nbytes = p.nocc * (p.nocc + 1) / 2 * p.nvir ** 2 * 8.0
+            from_g = {"shape": f"g ({nov}, {nov}) FP64", "kernel_ms": kms,
+                      "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / (kms * 1e-3) / 1e9,
+                      "hbm_frac": nbytes / (kms * 1e-3) / 1e9 / HBM_GBS,
+                      "E_total": sc_g.e_total, "dE_vs_fp64_fused": sc_g.e_total - e_ref}
+            log(f"energy from dense g {nov}^2: kernel {kms:.2f} ms, "
+                f"{from_g['achieved_GBs']:.0f} GB/s ({100 * from_g['hbm_frac']:.0f} % of HBM), "
+                f"dE vs fused FP64 {from_g['dE_vs_fp64_fused']:+.1e}")
+            # The same pass over per-pair blocks (contiguous nvir x nvir each, the
+            # layout of the domain fits): every i <= j block of g copied out.
+            from ferric_amd.dlpno import pair_energies_from_blocks
+
+            pairs = [(i, j) for i in range(p.nocc) for j in range(i, p.nocc)]
+            nv = p.nvir
+            blocks = torch.empty((len(pairs), nv, nv), dtype=torch.float64, device=dev)
+            for k, (i, j) in enumerate(pairs):
+                blocks[k].copy_(g[i * nv:(i + 1) * nv, j * nv:(j + 1) * nv])
+            del g
+            torch.cuda.synchronize()
+            pos, pss = pair_energies_from_blocks(blocks, pairs, p.eps, p.nocc, nv)
+            ks = []
+            for _ in range(max(3, args.steps)):
+                pair_energies_from_blocks(blocks, pairs, p.eps, p.nocc, nv)
+                ks.append(last_timing()["contract_ms"])
+            bms = float(np.median(ks))
+            from_g["pair_blocks"] = {"kernel_ms": bms,
+                                     "achieved_GBs": nbytes / (bms * 1e-3) / 1e9,
+                                     "hbm_frac": nbytes / (bms * 1e-3) / 1e9 / HBM_GBS,
+                                     "sum_minus_dense": float(np.sum(pos) + np.sum(pss))
+                                     - sc_g.e_total}
+            log(f"energy from {len(pairs)} per-pair blocks: kernel {bms:.2f} ms, "
+                f"{from_g['pair_blocks']['achieved_GBs']:.0f} GB/s")
+            del blocks
+            torch.cuda.empty_cache()
 
     # The other named shapes: water (5/19/84, launch-latency bound), decane scale
     # (41/209/960) in FP64 and FP32, and the decane-scale adjoint in FP64 and BF16x3.
@@ -544,6 +599,7 @@ def main():
         "energy_adjoint": adj,
         "batched_domains_512": batched,
         "dressing": dressing,
+        "energy_from_g": from_g,
         "small_configs": small,
         "roofline": {
             "bound": "mfma",

@@ -160,6 +160,12 @@ SIGNATURES = {
     "fh_domain_fit_pair_blocks_dev": (_c_int, [_c_vp, _c_vp, _c_sz, _c_sz, _c_sz, _c_dp, _c_dp,
                                                ctypes.c_double, _c_vp, _c_sz, _c_vp, _c_vp,
                                                ctypes.POINTER(_c_sz)]),
+    "fh_rimp2_energy_from_g": (_c_int, [_c_dp, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz, _c_sz,
+                                        ctypes.POINTER(SpinComponentsC)]),
+    "fh_rimp2_energy_from_g_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz, _c_sz,
+                                            _c_vp, ctypes.POINTER(SpinComponentsC)]),
+    "fh_pair_energies_from_blocks_dev": (_c_int, [_c_vp, _c_vp, _c_sz, _c_sz, _c_sz, _c_dp,
+                                                  _c_sz, _c_sz, _c_sz, _c_vp, _c_dp, _c_dp]),
     "fh_dress_b_ov_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz,
                                    _c_vp, _c_vp]),
     "fh_u_same_spin": (_c_int, [ctypes.POINTER(SpinChannelC), _c_sz, _c_int, _c_dp]),
@@ -178,7 +184,7 @@ SIGNATURES = {
 }
 
 _lib = None
-ABI_VERSION = 7   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 8   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

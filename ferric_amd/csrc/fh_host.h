@@ -43,6 +43,7 @@ enum SlotId {
     kSlotPnoMeta,    // PNO: pair list, per-pair results, masks, GEMM pointer arrays
     kSlotFitWork,    // domain fits: padded V_DD and [A_i | A_j] per batched pair
     kSlotFitMeta,    // domain fits: domain index lists, sizes, pairs, Cholesky info
+    kSlotGPairs,     // energies from explicit (ia|jb) blocks: the pair list
     kNumSlots
 };
 

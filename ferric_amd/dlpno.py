@@ -289,6 +289,34 @@ def domain_fit_pair_blocks(a, v, aux_xyz, occ_centers, nocc: int, nvir: int, rad
     return out, int(dmax.value)
 
 
+def pair_energies_from_blocks(blocks, pairs, eps, nocc: int, nvir: int, first_occ: int = 0,
+                              nocc_total: int | None = None):
+    """(e_os, e_ss) arrays over ``pairs`` from per-pair (ia|jb) blocks (a
+    contiguous float64 CUDA tensor (npairs, nvir, nvir), the layout of
+    :func:`domain_fit_pair_blocks`), the pair terms of spin_components_from_g
+    (rimp2.rs:984-1018) with the mirror factor applied. FP64."""
+    import torch
+
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    pl = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    npairs = pl.shape[0]
+    if (not _is_torch_cuda(blocks) or blocks.dtype != torch.float64
+            or not blocks.is_contiguous() or tuple(blocks.shape) != (npairs, nvir, nvir)):
+        raise LayoutError(f"blocks must be a contiguous float64 CUDA tensor ({npairs}, {nvir}, "
+                          f"{nvir})")
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    e_os, e_ss = np.zeros(npairs), np.zeros(npairs)
+    dp = ctypes.POINTER(ctypes.c_double)
+    stream = torch.cuda.current_stream(blocks.device).cuda_stream
+    st = lib().fh_pair_energies_from_blocks_dev(
+        ctypes.c_void_p(blocks.data_ptr()), ctypes.c_void_p(pl.ctypes.data), npairs, nocc, nvir,
+        e.ctypes.data_as(dp), e.size, first_occ, nocc_total, ctypes.c_void_p(stream),
+        e_os.ctypes.data_as(dp), e_ss.ctypes.data_as(dp))
+    check(st, "fh_pair_energies_from_blocks_dev")
+    return e_os, e_ss
+
+
 def dlpno_mp2_from_b_ov(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int,
                         domains: PairDomains, cfg: DlpnoConfig = DlpnoConfig(),
                         precision: Precision = Precision.FP64,

@@ -200,6 +200,38 @@ def energy_adjoint(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
     return sc, c_os * sc.e_os + c_ss * sc.e_ss, gamma, deps
 
 
+def spin_components_from_g(g, eps, nocc: int, nvir: int, first_occ: int = 0,
+                           nocc_total: int | None = None) -> SpinComponents:
+    """rimp2.rs:984-1018 on the device: the components from an explicit dense
+    (nocc*nvir) x (nocc*nvir) (ia|jb) matrix (the robust fit's non-Gram G).
+    ``g``: numpy array (host entry) or contiguous float64 CUDA tensor (device
+    entry on the current torch stream). FP64."""
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    nov = nocc * nvir
+    out = _lib.SpinComponentsC()
+    L = lib()
+    if _is_torch_cuda(g):
+        import torch
+
+        if g.dtype != torch.float64 or not g.is_contiguous() or tuple(g.shape) != (nov, nov):
+            raise LayoutError(f"g must be a contiguous float64 ({nov}, {nov}) tensor")
+        stream = torch.cuda.current_stream(g.device).cuda_stream
+        st = L.fh_rimp2_energy_from_g_dev(ctypes.c_void_p(g.data_ptr()), nocc, nvir,
+                                          e.ctypes.data_as(_lib._c_dp), e.size, first_occ,
+                                          nocc_total, ctypes.c_void_p(stream), ctypes.byref(out))
+    else:
+        gn = np.ascontiguousarray(np.asarray(g, dtype=np.float64))
+        if gn.shape != (nov, nov):
+            raise LayoutError(f"g is {gn.shape}, expected ({nov}, {nov})")
+        st = L.fh_rimp2_energy_from_g(gn.ctypes.data_as(_lib._c_dp), nocc, nvir,
+                                      e.ctypes.data_as(_lib._c_dp), e.size, first_occ,
+                                      nocc_total, ctypes.byref(out))
+    check(st, "fh_rimp2_energy_from_g")
+    return SpinComponents(out.e_os, out.e_ss, out.e_total)
+
+
 def _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, mixed_ok) -> SpinComponents:
     """dispatch_spin_components (rimp2.rs:1360-1375) minus the CPU branch: the
     reduced-precision kernel only when ``mixed_ok`` and the settings allow it."""

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 7
+#define FERRIC_HIP_ABI_VERSION 8
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -313,6 +313,33 @@ fh_status fh_domain_fit_pair_blocks_dev(const double *a_dev, const double *v_dev
                                         const double *centers, double radius,
                                         const int32_t *pairs, size_t npairs, void *stream,
                                         double *j_dev, size_t *max_domain);
+
+/* ---- Energies from explicit (ia|jb) blocks ------------------------------ */
+
+/* spin_components_from_g (crates/ferric-mp2/src/rimp2.rs:984-1018): the MP2
+ * components from a dense (nocc*nvir) x (nocc*nvir) row-major g, for integrals
+ * that are not a Gram product (the robust fit's three-term G); pair (i, j)
+ * reads (ia|jb) = g[i*nvir + a][j*nvir + b] and (ib|ja) = g[i*nvir + b][j*nvir
+ * + a]. One HBM pass over the i <= j blocks, FP64. Like the reference, no
+ * denominator is refused. The host entry uploads g (8 (nocc nvir)^2 bytes). */
+fh_status fh_rimp2_energy_from_g(const double *g, size_t nocc, size_t nvir, const double *eps,
+                                 size_t n_eps, size_t first_occ, size_t nocc_total,
+                                 fh_spin_components *out);
+fh_status fh_rimp2_energy_from_g_dev(const double *g_dev, size_t nocc, size_t nvir,
+                                     const double *eps, size_t n_eps, size_t first_occ,
+                                     size_t nocc_total, void *stream, fh_spin_components *out);
+
+/* Per-pair energies from per-pair blocks (the layout fh_domain_fit_pair_blocks
+ * writes): block k (nvir x nvir, row-major, device) holds (i_k a|j_k b) at
+ * [a][b]; e_os[k], e_ss[k] (host) are the pair's terms of
+ * spin_components_from_g, the mirror factor applied. No counterpart entry in
+ * the reference: it is the energy of its domain-fit integrals
+ * (lmp2_amplitude.rs:1666-1700) without the amplitude iterations. */
+fh_status fh_pair_energies_from_blocks_dev(const double *blocks_dev, const int32_t *pairs,
+                                           size_t npairs, size_t nocc, size_t nvir,
+                                           const double *eps, size_t n_eps, size_t first_occ,
+                                           size_t nocc_total, void *stream, double *e_os,
+                                           double *e_ss);
 
 /* ---- Dressing / MO transform upstream of the pair stage ---------------- */
 

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.fh_abi_version() == 7
+    assert L.fh_abi_version() == 8
 
 
 def test_enum_values_match_header():
@@ -70,6 +70,26 @@ def test_pno_entry_checks_before_the_device():
     assert call([(1, 0)], 0.0) == _lib.Status.LAYOUT
     assert call([(0, 2)], 0.0) == _lib.Status.LAYOUT
     assert call([(-1, 1)], 0.0) == _lib.Status.LAYOUT
+
+
+def test_from_g_entries_check_before_the_device():
+    eps = np.array([-1.0, -0.8, 0.5, 0.6, 0.7])
+    g = np.zeros((6, 6))
+    L = _lib.lib()
+    assert L.fh_rimp2_energy_from_g(g.ctypes.data_as(_lib._c_dp), 2, 3,
+                                    eps.ctypes.data_as(_lib._c_dp), 5, 0, 2,
+                                    None) == _lib.Status.LAYOUT
+    out = _lib.SpinComponentsC()
+    assert L.fh_rimp2_energy_from_g(g.ctypes.data_as(_lib._c_dp), 2, 3,
+                                    eps.ctypes.data_as(_lib._c_dp), 4, 0, 2,
+                                    ctypes.byref(out)) == _lib.Status.LAYOUT  # eps too short
+    os_, ss = np.zeros(1), np.zeros(1)
+    for pair in ((1, 0), (0, 2)):
+        pl = np.array(pair, dtype=np.int32)
+        assert L.fh_pair_energies_from_blocks_dev(
+            ctypes.c_void_p(0), ctypes.c_void_p(pl.ctypes.data), 1, 2, 3,
+            eps.ctypes.data_as(_lib._c_dp), 5, 0, 2, None, os_.ctypes.data_as(_lib._c_dp),
+            ss.ctypes.data_as(_lib._c_dp)) == _lib.Status.LAYOUT
 
 
 def test_domain_fit_entry_checks_before_the_device():
