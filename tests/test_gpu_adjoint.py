@@ -194,7 +194,7 @@ def test_autograd_function(golden):
     want_g, want_d = c["gamma_scs"], c["deps_scs"]   # (1.2, 1/3): the SCS weights
     assert np.max(np.abs(b.grad.cpu().numpy() - want_g)) <= 1e-12 * np.max(np.abs(want_g))
     assert np.max(np.abs(eps.grad.numpy() - want_d)) <= 1e-12 * np.max(np.abs(want_d))
-    assert float(e) == pytest.approx(1.2 * c["energy"][0] + c["energy"][1] / 3.0, rel=1e-12)
+    assert float(e.detach()) == pytest.approx(1.2 * c["energy"][0] + c["energy"][1] / 3.0, rel=1e-12)
 
 
 def test_adjoint_errors():
