@@ -380,6 +380,7 @@ def main():
     # The other named shapes: water (5/19/84, launch-latency bound), decane scale
     # (41/209/960) in FP64 and FP32, and the decane-scale adjoint in FP64 and BF16x3.
     small = {}
+    local_chain = None  # (system, retained pairs) of the local-MP2 leg, for the CPU sample
     if not args.no_small_configs:
         from ferric_amd.rimp2 import energy_adjoint
 
@@ -525,6 +526,7 @@ def main():
         loc["domain_fit_40x100_480aux_r6"] = {"ms_median": med, "ms_min": mn, "pairs": 820,
                                               "max_domain": dmax}
         small["local MP2"] = loc
+        local_chain = (ch, list(dom.pairs))
         log(f"local MP2 ({ch.name}): pair energies {loc['pair_energies_fp64']['ms_median']:.2f} "
             f"ms, default DLPNO {loc['dlpno_default']['ms_median']:.2f} ms ({len(dom.pairs)} "
             f"pairs, dE {loc['dlpno_default']['dE_vs_dense']:+.2e}), PNO 1e-6 "
@@ -576,6 +578,25 @@ def main():
                             "gpu_fp64_speedup": float(np.median(ts)) / gpu["ms_median"]}
             log(f"cpu {dc.name}: {np.median(ts):.1f} ms median, GPU FP64 - CPU "
                 f"{gpu['E_total'] - r[2]:+.2e} Eh")
+        if local_chain is not None:
+            # The PNO path on the host (the oracle's numpy restatement of the
+            # reference's per-pair loop) over the first 60 retained pairs of the
+            # chain, extrapolated to all of them.
+            import oracle
+
+            ch, kept = local_chain
+            sample = kept[:60]
+            t0 = time.perf_counter()
+            oracle.pno_pair_energies(ch.b_ov, ch.eps, ch.nocc, ch.nvir, 0, ch.nocc, sample, 1e-6)
+            dt = time.perf_counter() - t0
+            gpu_ms = small["local MP2"]["pno_1e-6"]["ms_median"]
+            est = dt * 1e3 * len(kept) / len(sample)
+            cpu["local MP2 PNO 1e-6"] = {"sample": f"{len(sample)} of {len(kept)} pairs of "
+                                                   f"{ch.name}", "sample_ms": dt * 1e3,
+                                         "ms_extrapolated": est,
+                                         "gpu_speedup": est / gpu_ms}
+            log(f"cpu PNO 1e-6: {dt * 1e3:.0f} ms for {len(sample)} pairs, ~{est:.0f} ms for "
+                f"{len(kept)} (GPU {gpu_ms:.0f} ms)")
 
     out = {
         "metric": "RI-MP2 time-to-E_corr + TFLOP/s, mixed vs FP64 on 1 GPU at |ΔE|≤1 µEh",
