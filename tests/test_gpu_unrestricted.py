@@ -137,3 +137,21 @@ def test_signed_device_entry_and_errors(oracle_mod):
         u.u_opposite_spin(ch, ch, Precision.FP8_E4M3)
     assert u.try_u_opposite_spin_on_device(ch, ch) == pytest.approx(
         oracle_mod.u_opposite_spin(_oracle_ch(ch), _oracle_ch(ch)), abs=1e-10)
+
+
+def test_same_spin_int8x4_within_the_format_bound(pair_mid, oracle_mod):
+    """The same-spin block through the INT8X4 kernel: half the closed-shell
+    same-spin sum, so within half of the format's derived same-spin bound of
+    the oracle (u_rimp2.rs:693-729), and FP64-class (1e-8 Eh)."""
+    from tests import quant
+
+    for ch in pair_mid:
+        print("uhf same-spin int8x4", ch.nocc, ch.nvir, flush=True)
+        got = u.u_same_spin(ch, Precision.INT8X4)
+        want = oracle_mod.u_same_spin(*_oracle_ch(ch))
+        _, bss = quant.int8x4_energy_bound(ch.b, ch.eps, ch.nocc, ch.nvir, ch.first_occ,
+                                           ch.nocc_total)
+        assert abs(got - want) <= 0.5 * bss + 1e-14, (got, want, bss)
+        assert abs(got - want) <= 1e-8
+    with pytest.raises(ferric_amd.UnsupportedError):
+        u.u_opposite_spin(pair_mid[0], pair_mid[1], Precision.INT8X4)
