@@ -86,27 +86,27 @@ __global__ __launch_bounds__(kThreads) void pno_symmetrize_kernel(int nvir, doub
 }
 
 // Selection (local_pno.rs:156-185): occupations n (ascending from the
-// eigensolver), eigenvectors W in the columns of w. Column k of Q is W_k when
-// |n_k| >= cut and zero otherwise; when nothing passes, the largest occupation
-// is kept (the first of equal ones in descending order, i.e. the lowest index
-// among the largest values). S = diag(e_vir) Q. Per pair: the kept count, the
-// discarded weight (sum of |n_k| over the discarded k, in descending order of
-// n) and the keep mask for pno_shift_kernel.
+// eigensolver), eigenvectors W in the columns of w. The kept PNOs are those
+// with |n_k| >= cut; when none passes, the largest occupation is kept (the
+// first of equal ones in descending order, i.e. the lowest index among the
+// largest values). They are written compacted, in descending order of n, to
+// the leading columns of q (the rest of q is zero), with their count and the
+// discarded weight (sum of |n_k| over the discarded k, in descending order).
 __global__ __launch_bounds__(kThreads) void pno_select_kernel(
     int nvir, double cut, const double *__restrict__ occ, const double *__restrict__ w,
-    const double *__restrict__ eps_vir, double *__restrict__ q, double *__restrict__ s,
-    int32_t *__restrict__ kept, double *__restrict__ discarded, int32_t *__restrict__ keepmask) {
+    double *__restrict__ q, int32_t *__restrict__ kept, double *__restrict__ discarded) {
     const int p = blockIdx.x;
     const int64_t mat = (int64_t)nvir * nvir;
     const double *n = occ + (int64_t)p * nvir;
     int mine = 0;
-    for (int k = threadIdx.x; k < nvir; k += kThreads) mine |= fabs(n[k]) >= cut ? 1 : 0;
+    for (int k = threadIdx.x; k < nvir; k += kThreads) mine += fabs(n[k]) >= cut ? 1 : 0;
     const int any = __syncthreads_or(mine);
     int top = -1;
     if (!any) {
         top = nvir - 1;
         while (top > 0 && n[top - 1] == n[nvir - 1]) --top;
     }
+    __shared__ int npno_s;
     if (threadIdx.x == 0) {
         int cnt = 0;
         double dw = 0.0;
@@ -116,51 +116,86 @@ __global__ __launch_bounds__(kThreads) void pno_select_kernel(
             else
                 dw += fabs(n[k]);
         }
+        npno_s = cnt;
         kept[p] = cnt;
         discarded[p] = dw;
     }
-    int32_t *mask = keepmask + (int64_t)p * nvir;
-    for (int k = threadIdx.x; k < nvir; k += kThreads) mask[k] = fabs(n[k]) >= cut || k == top;
+    __syncthreads();
+    const int npno = npno_s;
     const double *wp = w + p * mat;
     double *qp = q + p * mat;
-    double *sp = s + p * mat;
-    for (int64_t e = threadIdx.x; e < mat; e += kThreads) {
-        const int a = (int)(e % nvir), k = (int)(e / nvir);
-        const double v = (fabs(n[k]) >= cut || k == top) ? wp[e] : 0.0;
-        qp[e] = v;
-        sp[e] = eps_vir[a] * v;
+    // Column k of W goes to column slot(k) = #{kept j > k} of q; one wave per
+    // column, its lanes counting over j and copying the rows.
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    for (int k = wid; k < nvir; k += kThreads / kWave) {
+        if (!(fabs(n[k]) >= cut || k == top)) continue;  // uniform over the wave
+        int c = 0;
+        for (int j = k + 1 + lane; j < nvir; j += kWave) c += (fabs(n[j]) >= cut || j == top);
+        for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        double *dst = qp + (int64_t)c * nvir;
+        const double *src = wp + (int64_t)k * nvir;
+        for (int a = lane; a < nvir; a += kWave) dst[a] = src[a];
+    }
+    for (int64_t e = (int64_t)npno * nvir + threadIdx.x; e < mat; e += kThreads) qp[e] = 0.0;
+}
+
+// Gathers the pairs of one width class: qc[k] = the first c columns of q[idx[k]]
+// (nvir x c), sc[k] = diag(e_vir) qc[k].
+__global__ __launch_bounds__(kThreads) void pno_gather_kernel(
+    int nvir, int c, const int32_t *__restrict__ idx, const double *__restrict__ q,
+    const double *__restrict__ eps_vir, double *__restrict__ qc, double *__restrict__ sc) {
+    const int k = blockIdx.x;
+    const int64_t n = (int64_t)nvir * c;
+    const double *src = q + (int64_t)idx[k] * nvir * nvir;
+    for (int64_t e = threadIdx.x; e < n; e += kThreads) {
+        const double v = src[e];
+        qc[k * n + e] = v;
+        sc[k * n + e] = eps_vir[e % nvir] * v;
     }
 }
 
-// F[k][k] = shift for every discarded k (its row and column are exactly zero).
-__global__ __launch_bounds__(kThreads) void pno_shift_kernel(int nvir, double shift,
-                                                            const int32_t *__restrict__ keepmask,
+// F[j][j] = shift for the columns j >= kept of each pair's c x c block (their
+// rows and columns are exactly zero).
+__global__ __launch_bounds__(kThreads) void pno_shift_kernel(int c, double shift,
+                                                            const int32_t *__restrict__ idx,
+                                                            const int32_t *__restrict__ kept,
                                                             double *__restrict__ f) {
-    const int p = blockIdx.x;
-    const int32_t *mask = keepmask + (int64_t)p * nvir;
-    double *fp = f + (int64_t)p * nvir * nvir;
-    for (int k = threadIdx.x; k < nvir; k += kThreads)
-        if (!mask[k]) fp[k + (int64_t)k * nvir] = shift;
+    const int k = blockIdx.x;
+    double *fp = f + (int64_t)k * c * c;
+    for (int j = kept[idx[k]] + threadIdx.x; j < c; j += kThreads) fp[j + (int64_t)j * c] = shift;
 }
 
-// Energy in the semicanonical PNO basis (dlpno_mp2.rs:311-325): gt = g~
-// column-major, e~ the semicanonical energies; directions with e~ above
-// `skip` are the discarded ones and contribute nothing.
+// Device pointer arrays of X = g R for one class: g of pair idx[k], R and X
+// of class slot k (nvir x c each).
+__global__ void pno_class_pointers_kernel(int m, const int32_t *__restrict__ idx, const double *g,
+                                          int64_t mat, const double *r, double *x, int64_t vc,
+                                          const double **pa, const double **pb, double **pc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    pa[k] = g + (int64_t)idx[k] * mat;
+    pb[k] = r + (int64_t)k * vc;
+    pc[k] = x + (int64_t)k * vc;
+}
+
+// Energy in the semicanonical PNO basis (dlpno_mp2.rs:311-325) of the class's
+// pairs: gt = g~ (c x c, column-major), et the semicanonical energies;
+// directions with e~ above `skip` are padding and contribute nothing.
 __global__ __launch_bounds__(kThreads) void pno_energy_kernel(
-    const int32_t *__restrict__ pairs, const double *__restrict__ eps_occ, int nvir, double skip,
-    const double *__restrict__ et, const double *__restrict__ gt, double *__restrict__ out) {
-    const int p = blockIdx.x;
+    const int32_t *__restrict__ pairs, const int32_t *__restrict__ idx,
+    const double *__restrict__ eps_occ, int c, double skip, const double *__restrict__ et,
+    const double *__restrict__ gt, double *__restrict__ out) {
+    const int k = blockIdx.x, p = idx[k];
     const int i = pairs[2 * p], j = pairs[2 * p + 1];
     const double eij = eps_occ[i] + eps_occ[j];
-    const int64_t mat = (int64_t)nvir * nvir;
-    const double *e = et + (int64_t)p * nvir;
-    const double *g = gt + p * mat;
+    const int64_t mat = (int64_t)c * c;
+    const double *e = et + (int64_t)k * c;
+    const double *g = gt + k * mat;
     KSum os, ss;
     for (int64_t x = threadIdx.x; x < mat; x += kThreads) {
-        const int a = (int)(x % nvir), b = (int)(x / nvir);
+        const int a = (int)(x % c), b = (int)(x / c);
         if (e[a] > skip || e[b] > skip) continue;
         const double d = eij - e[a] - e[b];
-        const double gab = g[x], gba = g[b + (int64_t)a * nvir];
+        const double gab = g[x], gba = g[b + (int64_t)a * c];
         os.add(gab * gab / d);
         ss.add(gab * (gab - gba) / d);
     }
@@ -185,6 +220,16 @@ __global__ __launch_bounds__(kThreads) void pno_energy_kernel(
         out[2 * p] = f * o.value();
         out[2 * p + 1] = f * s.value();
     }
+}
+
+// Width class of a pair with n kept PNOs: the smallest of 128, 256, 512, ...
+// that holds n, or nvir. Below 128 the batched divide-and-conquer eigensolver
+// was measured slower than at 128 (n = 64 took 3-13x the time of n = 128 for
+// the same pairs), so no class is narrower.
+int pno_class(int n, int nvir) {
+    int c = 128;
+    while (c < n) c *= 2;
+    return std::min(c, nvir);
 }
 
 // Pairs per batch: the five nvir x nvir matrices of a pair in about 4 GiB
@@ -232,8 +277,8 @@ fh_status pno_core(State &s, const double *b, size_t naux, size_t nocc, size_t n
     fh_status st = reserve(
         s, {{kSlotPnoWork, sizeof(double) * nb * (5 * mat + small)},
             {kSlotPnoMeta, sizeof(int32_t) * 2 * npairs + sizeof(double) * 3 * npairs +
-                               sizeof(int32_t) * (npairs + nb * nvir + 2 * nb) +
-                               3 * sizeof(void *) * nb + sizeof(double) * (nocc + nvir) + 256}});
+                               sizeof(int32_t) * (npairs + 3 * nb) + 3 * sizeof(void *) * nb +
+                               sizeof(double) * (nocc + nvir) + 256}});
     if (st != FH_OK) return st;
     double *G = (double *)s.slots[kSlotPnoWork].ptr;
     double *M = G + nb * mat;
@@ -249,10 +294,10 @@ fh_status pno_core(State &s, const double *b, size_t naux, size_t nocc, size_t n
     double *deps = dw + npairs;
     int32_t *dpairs = (int32_t *)(deps + nocc + nvir);
     int32_t *kept = dpairs + 2 * npairs;
-    int32_t *mask = kept + npairs;
-    int32_t *info = mask + nb * nvir;  // first eigensolve
-    int32_t *info2 = info + nb;         // second eigensolve
-    const size_t off = (size_t)((char *)(info2 + nb) - mp);
+    int32_t *info = kept + npairs;  // first eigensolve, per pair of the batch
+    int32_t *info2 = info + nb;     // second eigensolve, per pair of a class
+    int32_t *cls = info2 + nb;      // batch indices of the pairs of a class
+    const size_t off = (size_t)((char *)(cls + nb) - mp);
     const double **pa = (const double **)(mp + (off + 7) / 8 * 8);
     const double **pb = pa + nb;
     double **pc = (double **)(pb + nb);
@@ -278,7 +323,7 @@ fh_status pno_core(State &s, const double *b, size_t naux, size_t nocc, size_t n
     const int V = (int)nvir, nov = (int)(nocc * nvir);
     const rocblas_stride sm = (rocblas_stride)mat;
     const auto N = rocblas_operation_none, T = rocblas_operation_transpose;
-    std::vector<int32_t> hinfo(2 * nb);
+    std::vector<int32_t> hinfo(nb);
     mark(s, 0, stream);
     mark(s, 1, stream);
     for (size_t p0 = 0; p0 < npairs; p0 += nb) {
@@ -309,48 +354,86 @@ fh_status pno_core(State &s, const double *b, size_t naux, size_t nocc, size_t n
                      "PNO eigensolver");
         if (st != FH_OK) return st;
         hipLaunchKernelGGL(pno_select_kernel, dim3(n), dim3(kThreads), 0, stream, V, t_cut_pno,
-                           Wd, M, deps + nocc, Q, S, kept + p0, dw + p0, mask);
-        // F = Q^T diag(e_vir) Q
-        st = blas_ok(rocblas_dgemm_strided_batched(h, T, N, V, V, V, &one, Q, V, sm, S, V, sm,
-                                                   &zero, F, V, sm, n),
-                     "PNO Fock GEMM");
-        if (st != FH_OK) return st;
-        hipLaunchKernelGGL(pno_shift_kernel, dim3(n), dim3(kThreads), 0, stream, V, shift, mask,
-                           F);
-        st = blas_ok(rocsolver_dsyevd_strided_batched(h, rocblas_evect_original,
-                                                      rocblas_fill_upper, V, F, V, sm, We, V,
-                                                      Ex, V, info2, n),
-                     "PNO semicanonical eigensolver");
-        if (st != FH_OK) return st;
-        // R = Q U (into S); X = g R (into M); g~ = R^T X (into Q)
-        st = blas_ok(rocblas_dgemm_strided_batched(h, N, N, V, V, V, &one, Q, V, sm, F, V, sm,
-                                                   &zero, S, V, sm, n),
-                     "PNO rotation GEMM");
-        if (st != FH_OK) return st;
-        st = blas_ok(rocblas_dgemm_strided_batched(h, N, N, V, V, V, &one, G, V, sm, S, V, sm,
-                                                   &zero, M, V, sm, n),
-                     "PNO transform GEMM 1");
-        if (st != FH_OK) return st;
-        st = blas_ok(rocblas_dgemm_strided_batched(h, T, N, V, V, V, &one, S, V, sm, M, V, sm,
-                                                   &zero, Q, V, sm, n),
-                     "PNO transform GEMM 2");
-        if (st != FH_OK) return st;
-        hipLaunchKernelGGL(pno_energy_kernel, dim3(n), dim3(kThreads), 0, stream, bp, deps, V,
-                           skip, We, Q, out + 2 * p0);
+                           Wd, M, Q, kept + p0, dw + p0);
         st = hip_check(hipGetLastError(), "PNO kernels");
         if (st != FH_OK) return st;
-        s.pending.kernel_launches += 14;
-        st = hip_check(hipMemcpyAsync(hinfo.data(), info, sizeof(int32_t) * 2 * nb,
+        // The kept counts set each pair's width class: the rest of the pair's
+        // work runs at c x c (c >= kept) instead of nvir x nvir.
+        std::vector<int32_t> hk(n);
+        st = hip_check(hipMemcpyAsync(hk.data(), kept + p0, sizeof(int32_t) * n,
+                                      hipMemcpyDeviceToHost, stream),
+                       "D2H kept counts");
+        if (st != FH_OK) return st;
+        st = hip_check(hipMemcpyAsync(hinfo.data(), info, sizeof(int32_t) * n,
                                       hipMemcpyDeviceToHost, stream),
                        "D2H eigensolver info");
         if (st != FH_OK) return st;
         st = hip_check(hipStreamSynchronize(stream), "stream sync");
         if (st != FH_OK) return st;
         for (int k = 0; k < n; ++k)
-            if (hinfo[k] != 0 || hinfo[nb + k] != 0)
-                return fail(FH_KERNEL, "PNO eigensolver failed for pair (%d, %d): info %d, %d",
-                            pairs[2 * (p0 + k)], pairs[2 * (p0 + k) + 1], hinfo[k],
-                            hinfo[nb + k]);
+            if (hinfo[k] != 0)
+                return fail(FH_KERNEL, "PNO eigensolver failed for pair (%d, %d): info %d",
+                            pairs[2 * (p0 + k)], pairs[2 * (p0 + k) + 1], hinfo[k]);
+        std::vector<std::vector<int32_t>> byc(V + 1);
+        for (int k = 0; k < n; ++k) byc[pno_class(hk[k], V)].push_back(k);
+        for (int c = 1; c <= V; ++c) {
+            const std::vector<int32_t> &idx = byc[c];
+            const int m = (int)idx.size();
+            if (m == 0) continue;
+            const rocblas_stride svc = (rocblas_stride)V * c, scc = (rocblas_stride)c * c;
+            st = hip_check(hipMemcpyAsync(cls, idx.data(), sizeof(int32_t) * m,
+                                          hipMemcpyHostToDevice, stream),
+                           "H2D class");
+            if (st != FH_OK) return st;
+            // QC (in M) and SC = diag(e_vir) QC (in S), nvir x c per pair
+            hipLaunchKernelGGL(pno_gather_kernel, dim3(m), dim3(kThreads), 0, stream, V, c, cls,
+                               Q, deps + nocc, M, S);
+            // F = QC^T diag(e_vir) QC (in F), c x c
+            st = blas_ok(rocblas_dgemm_strided_batched(h, T, N, c, c, V, &one, M, V, svc, S, V,
+                                                       svc, &zero, F, c, scc, m),
+                         "PNO Fock GEMM");
+            if (st != FH_OK) return st;
+            hipLaunchKernelGGL(pno_shift_kernel, dim3(m), dim3(kThreads), 0, stream, c, shift,
+                               cls, kept + p0, F);
+            st = blas_ok(rocsolver_dsyevd_strided_batched(h, rocblas_evect_original,
+                                                          rocblas_fill_upper, c, F, c, scc, We,
+                                                          c, Ex, c, info2, m),
+                         "PNO semicanonical eigensolver");
+            if (st != FH_OK) return st;
+            // R = QC U (into S); X = g R (into M); g~ = R^T X (into F)
+            st = blas_ok(rocblas_dgemm_strided_batched(h, N, N, V, c, c, &one, M, V, svc, F, c,
+                                                       scc, &zero, S, V, svc, m),
+                         "PNO rotation GEMM");
+            if (st != FH_OK) return st;
+            hipLaunchKernelGGL(pno_class_pointers_kernel, dim3((m + 63) / 64), dim3(64), 0,
+                               stream, m, cls, G, (int64_t)mat, S, M, (int64_t)svc, pa, pb, pc);
+            st = blas_ok(rocblas_dgemm_batched(h, N, N, V, c, V, &one, pa, V, pb, V, &zero, pc,
+                                               V, m),
+                         "PNO transform GEMM 1");
+            if (st != FH_OK) return st;
+            st = blas_ok(rocblas_dgemm_strided_batched(h, T, N, c, c, V, &one, S, V, svc, M, V,
+                                                       svc, &zero, F, c, scc, m),
+                         "PNO transform GEMM 2");
+            if (st != FH_OK) return st;
+            hipLaunchKernelGGL(pno_energy_kernel, dim3(m), dim3(kThreads), 0, stream, bp, cls,
+                               deps, c, skip, We, F, out + 2 * p0);
+            st = hip_check(hipGetLastError(), "PNO kernels");
+            if (st != FH_OK) return st;
+            s.pending.kernel_launches += 9;
+            std::vector<int32_t> hi2(m);
+            st = hip_check(hipMemcpyAsync(hi2.data(), info2, sizeof(int32_t) * m,
+                                          hipMemcpyDeviceToHost, stream),
+                           "D2H eigensolver info");
+            if (st != FH_OK) return st;
+            st = hip_check(hipStreamSynchronize(stream), "stream sync");
+            if (st != FH_OK) return st;
+            for (int k = 0; k < m; ++k)
+                if (hi2[k] != 0)
+                    return fail(FH_KERNEL,
+                                "PNO semicanonical eigensolver failed for pair (%d, %d): info %d",
+                                pairs[2 * (p0 + idx[k])], pairs[2 * (p0 + idx[k]) + 1], hi2[k]);
+        }
+        s.pending.kernel_launches += 8;
     }
     mark(s, 2, stream);
     std::vector<double> ho(3 * npairs);
