@@ -166,6 +166,12 @@ SIGNATURES = {
                                             _c_vp, ctypes.POINTER(SpinComponentsC)]),
     "fh_pair_energies_from_blocks_dev": (_c_int, [_c_vp, _c_vp, _c_sz, _c_sz, _c_sz, _c_dp,
                                                   _c_sz, _c_sz, _c_sz, _c_vp, _c_dp, _c_dp]),
+    "fh_pair_energies_from_g_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz, _c_sz,
+                                             _c_vp, _c_dp, _c_dp]),
+    "fh_dlpno_pno_pair_energies_from_g_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz,
+                                                       _c_sz, _c_vp, _c_sz, ctypes.c_double,
+                                                       _c_vp, _c_dp, _c_dp,
+                                                       ctypes.POINTER(PnoDiagnosticsC)]),
     "fh_dress_b_ov_dev": (_c_int, [_c_vp, _c_sz, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz, _c_vp, _c_sz,
                                    _c_vp, _c_vp]),
     "fh_u_same_spin": (_c_int, [ctypes.POINTER(SpinChannelC), _c_sz, _c_int, _c_dp]),
@@ -184,7 +190,7 @@ SIGNATURES = {
 }
 
 _lib = None
-ABI_VERSION = 8   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 9   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

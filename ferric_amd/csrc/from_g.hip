@@ -360,3 +360,53 @@ extern "C" fh_status fh_pair_energies_from_blocks_dev(const double *blocks_dev,
     commit(s, Prec::F64, false, false);
     return FH_OK;
 }
+
+extern "C" fh_status fh_pair_energies_from_g_dev(const double *g_dev, size_t nocc, size_t nvir,
+                                                 const double *eps, size_t n_eps,
+                                                 size_t first_occ, size_t nocc_total,
+                                                 void *stream, double *e_os, double *e_ss) {
+    State &s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    begin(s);
+    if (!e_os || !e_ss) return fail(FH_LAYOUT, "null output pointer");
+    bool empty = false;
+    fh_status st = check_shapes(1, nocc, nvir, n_eps, first_occ, nocc_total, &empty);
+    if (st != FH_OK) return st;
+    if (empty) {
+        for (size_t k = 0; k < nocc * nocc; ++k) e_os[k] = e_ss[k] = 0.0;
+        commit(s, Prec::F64, false, false);
+        return FH_OK;
+    }
+    if (!g_dev || !eps) return fail(FH_LAYOUT, "null input pointer");
+    // estimate_pair_energies refuses a non-negative denominator
+    // (pair_energy_screen.rs:160-167): the largest is 2 max e_occ - 2 min e_vir.
+    double emax = -INFINITY, vmin = INFINITY;
+    for (size_t i = 0; i < nocc; ++i) emax = std::max(emax, eps[first_occ + i]);
+    for (size_t a = 0; a < nvir; ++a) vmin = std::min(vmin, eps[nocc_total + a]);
+    if (!(2.0 * emax - 2.0 * vmin < 0.0))
+        return fail(FH_LAYOUT,
+                    "non-negative MP2 denominator %g (max eps_occ %g, min eps_vir %g): the "
+                    "reference has a zero or inverted gap",
+                    2.0 * emax - 2.0 * vmin, emax, vmin);
+    st = ensure_device(s);
+    if (st != FH_OK) return st;
+    std::vector<int32_t> pairs;
+    pairs.reserve(nocc * (nocc + 1));
+    for (size_t i = 0; i < nocc; ++i)
+        for (size_t j = i; j < nocc; ++j) {
+            pairs.push_back((int32_t)i);
+            pairs.push_back((int32_t)j);
+        }
+    std::vector<double> po;
+    st = from_g_core(s, g_dev, true, pairs.data(), pairs.size() / 2, nocc, nvir, eps, first_occ,
+                     nocc_total, (hipStream_t)stream, po);
+    if (st != FH_OK) return st;
+    size_t p = 0;
+    for (size_t i = 0; i < nocc; ++i)
+        for (size_t j = i; j < nocc; ++j, ++p) {
+            e_os[i * nocc + j] = e_os[j * nocc + i] = po[2 * p];
+            e_ss[i * nocc + j] = e_ss[j * nocc + i] = po[2 * p + 1];
+        }
+    commit(s, Prec::F64, false, false);
+    return FH_OK;
+}

@@ -56,6 +56,23 @@ __global__ void pno_pointers_kernel(const int32_t *__restrict__ pairs, int np, c
     pc[p] = g0 + (int64_t)p * mat;
 }
 
+// g of each pair from a dense row-major (nocc nvir)^2 matrix (the reference's
+// dense-g entry, dlpno_mp2.rs:354-420): G[p][a + b nvir] = g[i nvir + a][j nvir + b].
+__global__ __launch_bounds__(256) void pno_gather_g_kernel(const int32_t *__restrict__ pairs,
+                                                           const double *__restrict__ g,
+                                                           int64_t nov, int nvir,
+                                                           double *__restrict__ out) {
+    const int p = blockIdx.x;
+    const int64_t mat = (int64_t)nvir * nvir;
+    const double *src = g + (int64_t)pairs[2 * p] * nvir * nov + (int64_t)pairs[2 * p + 1] * nvir;
+    double *dst = out + p * mat;
+    // b runs fastest: the reads follow g's rows
+    for (int64_t e = threadIdx.x; e < mat; e += 256) {
+        const int a = (int)(e / nvir), b = (int)(e % nvir);
+        dst[a + (int64_t)b * nvir] = src[(int64_t)a * nov + b];
+    }
+}
+
 // T = g / D (column-major, element (a, b) at a + b nvir), one workgroup per pair.
 __global__ __launch_bounds__(kThreads) void pno_amplitude_kernel(
     const int32_t *__restrict__ pairs, const double *__restrict__ eps_occ,
@@ -244,7 +261,9 @@ size_t pno_batch(size_t npairs, size_t nvir) {
     return std::min(nb, npairs);
 }
 
-fh_status pno_core(State &s, const double *b, size_t naux, size_t nocc, size_t nvir,
+// b: B (naux, nocc nvir) row-major, or, when dense_g, the dense g of
+// spin_components_from_g ((nocc nvir)^2 row-major; naux unused).
+fh_status pno_core(State &s, const double *b, bool dense_g, size_t naux, size_t nocc, size_t nvir,
                    const double *eps, size_t first_occ, size_t nocc_total,
                    const int32_t *pairs, size_t npairs, double t_cut_pno, hipStream_t stream,
                    double *e_os, double *e_ss, fh_pno_diagnostics *diag) {
@@ -329,13 +348,18 @@ fh_status pno_core(State &s, const double *b, size_t naux, size_t nocc, size_t n
     for (size_t p0 = 0; p0 < npairs; p0 += nb) {
         const int n = (int)std::min(nb, npairs - p0);
         const int32_t *bp = dpairs + 2 * p0;
-        hipLaunchKernelGGL(pno_pointers_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, bp, n,
-                           b, V, G, (int64_t)mat, pa, pb, pc);
-        // g = B_i^T B_j: column-major, B is (nov x naux) with leading dimension nov.
-        st = blas_ok(rocblas_dgemm_batched(h, N, T, V, V, (int)naux, &one, pa, nov, pb, nov,
-                                           &zero, pc, V, n),
-                     "PNO g GEMM");
-        if (st != FH_OK) return st;
+        if (dense_g) {
+            hipLaunchKernelGGL(pno_gather_g_kernel, dim3(n), dim3(256), 0, stream, bp, b,
+                               (int64_t)nov, V, G);
+        } else {
+            hipLaunchKernelGGL(pno_pointers_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, bp,
+                               n, b, V, G, (int64_t)mat, pa, pb, pc);
+            // g = B_i^T B_j: column-major, B is (nov x naux) with leading dimension nov.
+            st = blas_ok(rocblas_dgemm_batched(h, N, T, V, V, (int)naux, &one, pa, nov, pb, nov,
+                                               &zero, pc, V, n),
+                         "PNO g GEMM");
+            if (st != FH_OK) return st;
+        }
         hipLaunchKernelGGL(pno_amplitude_kernel, dim3(n), dim3(kThreads), 0, stream, bp, deps,
                            deps + nocc, V, G, Q);
         // D = T T^T + T^T T
@@ -515,7 +539,7 @@ extern "C" fh_status fh_dlpno_pno_pair_energies_dev(
     if (!b_dev) return fail(FH_LAYOUT, "null input pointer");
     st = ensure_device(s);
     if (st != FH_OK) return st;
-    st = pno_core(s, b_dev, naux, nocc, nvir, eps, first_occ, nocc_total, pairs, npairs,
+    st = pno_core(s, b_dev, false, naux, nocc, nvir, eps, first_occ, nocc_total, pairs, npairs,
                   t_cut_pno, (hipStream_t)stream, e_os, e_ss, diag);
     if (st != FH_OK) return st;
     commit(s, Prec::F64, false, false);
@@ -554,8 +578,37 @@ extern "C" fh_status fh_dlpno_pno_pair_energies(const double *b_ov, size_t naux,
                    "H2D b_ov");
     if (st != FH_OK) return st;
     s.pending.h2d_bytes += bbytes;
-    st = pno_core(s, (const double *)s.slots[kSlotStage].ptr, naux, nocc, nvir, eps, first_occ,
-                  nocc_total, pairs, npairs, t_cut_pno, nullptr, e_os, e_ss, diag);
+    st = pno_core(s, (const double *)s.slots[kSlotStage].ptr, false, naux, nocc, nvir, eps,
+                  first_occ, nocc_total, pairs, npairs, t_cut_pno, nullptr, e_os, e_ss, diag);
+    if (st != FH_OK) return st;
+    commit(s, Prec::F64, false, false);
+    return FH_OK;
+}
+
+extern "C" fh_status fh_dlpno_pno_pair_energies_from_g_dev(
+    const double *g_dev, size_t nocc, size_t nvir, const double *eps, size_t n_eps,
+    size_t first_occ, size_t nocc_total, const int32_t *pairs, size_t npairs, double t_cut_pno,
+    void *stream, double *e_os, double *e_ss, fh_pno_diagnostics *diag) {
+    State &s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    clear_error();
+    s.stats.calls++;
+    s.pending = Pending{};
+    bool empty = false;
+    fh_status st = pno_checks(1, nocc, nvir, eps, n_eps, first_occ, nocc_total, pairs, npairs,
+                              t_cut_pno, e_os, e_ss, &empty);
+    if (st != FH_OK) return st;
+    if (diag) *diag = fh_pno_diagnostics{0, 0.0};
+    if (empty || npairs == 0) {
+        for (size_t k = 0; k < npairs; ++k) e_os[k] = e_ss[k] = 0.0;
+        commit(s, Prec::F64, false, false);
+        return FH_OK;
+    }
+    if (!g_dev) return fail(FH_LAYOUT, "null input pointer");
+    st = ensure_device(s);
+    if (st != FH_OK) return st;
+    st = pno_core(s, g_dev, true, 0, nocc, nvir, eps, first_occ, nocc_total, pairs, npairs,
+                  t_cut_pno, (hipStream_t)stream, e_os, e_ss, diag);
     if (st != FH_OK) return st;
     commit(s, Prec::F64, false, false);
     return FH_OK;

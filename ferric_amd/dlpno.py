@@ -15,6 +15,9 @@ crates/ferric-mp2/src/pair_energy_screen.rs:
   its eigenvectors above ``t_cut_pno`` (local_pno.rs:132-200), semicanonical
   rotation and the energy in that basis (dlpno_mp2.rs:249-325), batched over
   pairs on the device;
+* :func:`estimate_pair_energies`, :func:`pno_pair_energies_from_g` and
+  :func:`dlpno_mp2_spin_components`: the same from an explicit dense (ia|jb)
+  matrix (pair_energy_screen.rs:125-190, dlpno_mp2.rs:354-440);
 * :func:`domain_fit_pair_blocks` (``fh_domain_fit_pair_blocks[_dev]``): the
   domain-local RI fit J_ij = A_{i,D}^T V_DD^-1 A_{j,D} of each pair over the aux
   functions near either of its centres (lmp2_amplitude.rs:1707-1758);
@@ -317,6 +320,82 @@ def pair_energies_from_blocks(blocks, pairs, eps, nocc: int, nvir: int, first_oc
     return e_os, e_ss
 
 
+def _dense_g(g, nocc, nvir):
+    import torch
+
+    nov = nocc * nvir
+    if (not _is_torch_cuda(g) or g.dtype != torch.float64 or not g.is_contiguous()
+            or tuple(g.shape) != (nov, nov)):
+        raise LayoutError(f"g must be a contiguous float64 CUDA tensor ({nov}, {nov})")
+    return ctypes.c_void_p(torch.cuda.current_stream(g.device).cuda_stream)
+
+
+def estimate_pair_energies(g, eps, nocc: int, nvir: int, first_occ: int = 0,
+                           nocc_total: int | None = None) -> PairEnergies:
+    """pair_energy_screen.rs:125-190 on the device: the pair energies of an
+    explicit dense (ia|jb) matrix ``g`` (a CUDA tensor, not necessarily B^T B)."""
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    stream = _dense_g(g, nocc, nvir)
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    e_os, e_ss = np.zeros((nocc, nocc)), np.zeros((nocc, nocc))
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = lib().fh_pair_energies_from_g_dev(ctypes.c_void_p(g.data_ptr()), nocc, nvir,
+                                           e.ctypes.data_as(dp), e.size, first_occ, nocc_total,
+                                           stream, e_os.ctypes.data_as(dp),
+                                           e_ss.ctypes.data_as(dp))
+    check(st, "fh_pair_energies_from_g_dev")
+    return PairEnergies(nocc, e_os + e_ss, e_os, e_ss)
+
+
+def pno_pair_energies_from_g(g, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int,
+                             pairs, t_cut_pno: float):
+    """:func:`pno_pair_energies` with each pair's block read from a dense device
+    ``g`` (dlpno_mp2.rs:390-430)."""
+    stream = _dense_g(g, nocc, nvir)
+    pl = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    e_os, e_ss = np.zeros(pl.shape[0]), np.zeros(pl.shape[0])
+    diag = PnoDiagnosticsC()
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = lib().fh_dlpno_pno_pair_energies_from_g_dev(
+        ctypes.c_void_p(g.data_ptr()), nocc, nvir, e.ctypes.data_as(dp), e.size, first_occ,
+        nocc_total, ctypes.c_void_p(pl.ctypes.data), pl.shape[0], float(t_cut_pno), stream,
+        e_os.ctypes.data_as(dp), e_ss.ctypes.data_as(dp), ctypes.byref(diag))
+    check(st, "fh_dlpno_pno_pair_energies_from_g_dev")
+    return e_os, e_ss, int(diag.kept_pnos), float(diag.max_discarded_weight)
+
+
+def _sum_pairs(domains, nvir, pos, pss, kept, worst):
+    e_os = e_ss = 0.0
+    for k in range(len(domains.pairs)):  # the reference's order of accumulation
+        e_os += pos[k]
+        e_ss += pss[k]
+    n = len(domains.pairs)
+    vret = 1.0 if n == 0 or nvir == 0 else kept / (n * nvir)
+    return (SpinComponents(e_os, e_ss, e_os + e_ss),
+            DlpnoDiagnostics(domains.pair_retention(), vret, worst, n))
+
+
+def dlpno_mp2_spin_components(g, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int,
+                              domains: PairDomains, cfg: DlpnoConfig = DlpnoConfig()):
+    """dlpno_mp2.rs:354-440: (SpinComponents, DlpnoDiagnostics) over the retained
+    pairs of ``domains`` from an explicit dense ``g`` (a CUDA tensor). FP64."""
+    if cfg.t_cut_pno < 0.0:
+        raise LayoutError(f"t_cut_pno must be >= 0 (got {cfg.t_cut_pno}); use 0.0 to disable "
+                          "truncation")
+    if domains.nocc != nocc:
+        raise LayoutError(f"domains were built for nocc={domains.nocc}, but nocc={nocc}")
+    if not cfg.t_cut_pno == 0.0:
+        return _sum_pairs(domains, nvir, *pno_pair_energies_from_g(
+            g, eps, nocc, nvir, first_occ, nocc_total, domains.pairs, cfg.t_cut_pno))
+    pe = estimate_pair_energies(g, eps, nocc, nvir, first_occ, nocc_total)
+    e_os = math.fsum(pe.e_os[i, j] for i, j in domains.pairs)
+    e_ss = math.fsum(pe.e_ss[i, j] for i, j in domains.pairs)
+    return (SpinComponents(e_os, e_ss, e_os + e_ss),
+            DlpnoDiagnostics(domains.pair_retention(), 1.0, 0.0, len(domains.pairs)))
+
+
 def dlpno_mp2_from_b_ov(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_total: int,
                         domains: PairDomains, cfg: DlpnoConfig = DlpnoConfig(),
                         precision: Precision = Precision.FP64,
@@ -333,16 +412,8 @@ def dlpno_mp2_from_b_ov(b_ov, eps, nocc: int, nvir: int, first_occ: int, nocc_to
     if not cfg.t_cut_pno == 0.0:
         if precision != Precision.FP64:
             raise UnsupportedError("PNO truncation runs in FP64")
-        pos, pss, kept, worst = pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total,
-                                                  domains.pairs, cfg.t_cut_pno)
-        e_os = e_ss = 0.0
-        for k in range(len(domains.pairs)):  # the reference's order of accumulation
-            e_os += pos[k]
-            e_ss += pss[k]
-        n = len(domains.pairs)
-        vret = 1.0 if n == 0 or nvir == 0 else kept / (n * nvir)
-        return (SpinComponents(e_os, e_ss, e_os + e_ss),
-                DlpnoDiagnostics(domains.pair_retention(), vret, worst, n))
+        return _sum_pairs(domains, nvir, *pno_pair_energies(
+            b_ov, eps, nocc, nvir, first_occ, nocc_total, domains.pairs, cfg.t_cut_pno))
     pe = pair_energies
     if pe is None:
         pe = estimate_pair_energies_from_b_ov(b_ov, eps, nocc, nvir, first_occ, nocc_total,

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 8
+#define FERRIC_HIP_ABI_VERSION 9
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -340,6 +340,27 @@ fh_status fh_pair_energies_from_blocks_dev(const double *blocks_dev, const int32
                                            const double *eps, size_t n_eps, size_t first_occ,
                                            size_t nocc_total, void *stream, double *e_os,
                                            double *e_ss);
+
+/* estimate_pair_energies (crates/ferric-mp2/src/pair_energy_screen.rs:125-190)
+ * on a dense device g: the per-pair terms of fh_rimp2_energy_from_g as host
+ * nocc x nocc symmetric e_os, e_ss (the mirror factor applied). A
+ * non-negative denominator is FH_LAYOUT, as the reference refuses it. */
+fh_status fh_pair_energies_from_g_dev(const double *g_dev, size_t nocc, size_t nvir,
+                                      const double *eps, size_t n_eps, size_t first_occ,
+                                      size_t nocc_total, void *stream, double *e_os,
+                                      double *e_ss);
+
+/* The pair loop of dlpno_mp2_spin_components (dlpno_mp2.rs:354-440): PNO
+ * truncation as fh_dlpno_pno_pair_energies with each pair's (ia|jb) block read
+ * from a dense device g ((nocc*nvir)^2 row-major, not necessarily a Gram
+ * product) instead of formed from B. The ABI version is 9 since this entry
+ * and fh_pair_energies_from_g_dev were added. */
+fh_status fh_dlpno_pno_pair_energies_from_g_dev(const double *g_dev, size_t nocc, size_t nvir,
+                                                const double *eps, size_t n_eps,
+                                                size_t first_occ, size_t nocc_total,
+                                                const int32_t *pairs, size_t npairs,
+                                                double t_cut_pno, void *stream, double *e_os,
+                                                double *e_ss, fh_pno_diagnostics *diag);
 
 /* ---- Dressing / MO transform upstream of the pair stage ---------------- */
 

@@ -345,7 +345,32 @@ def pair_energies(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None):
     return e_os, e_ss
 
 
-def pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total, pairs, t_cut_pno):
+def pair_energies_from_g(g, eps, nocc, nvir, first_occ=0, nocc_total=None):
+    """estimate_pair_energies (crates/ferric-mp2/src/pair_energy_screen.rs:125-190)
+    split into (e_os, e_ss) like pair_energies, from an explicit dense g:
+    (ia|jb) = g[i nvir + a, j nvir + b], (ib|ja) = g[i nvir + b, j nvir + a].
+    A non-negative denominator raises ValueError."""
+    gg = _f64(g)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    eo = e[first_occ:first_occ + nocc]
+    ev = e[nocc_total:nocc_total + nvir]
+    e_os = np.zeros((nocc, nocc))
+    e_ss = np.zeros((nocc, nocc))
+    for i in range(nocc):
+        for j in range(i, nocc):
+            blk = gg[i * nvir:(i + 1) * nvir, j * nvir:(j + 1) * nvir]
+            d = eo[i] + eo[j] - ev[:, None] - ev[None, :]
+            if np.any(d >= 0.0):
+                raise ValueError(f"non-negative MP2 denominator for pair ({i}, {j})")
+            fac = 1.0 if i == j else 2.0
+            e_os[i, j] = e_os[j, i] = fac * float(np.sum(blk * blk / d))
+            e_ss[i, j] = e_ss[j, i] = fac * float(np.sum(blk * (blk - blk.T) / d))
+    return e_os, e_ss
+
+
+def pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total, pairs, t_cut_pno, g=None):
     """The pair loop of dlpno_mp2_from_b_ov (crates/ferric-mp2/src/dlpno_mp2.rs:718-744)
     per pair: amplitudes T = g / D (:700-712), pno_for_pair (local_pno.rs:132-200:
     D = sym(T T^T + T^T T), eigh, keep |n| >= t_cut_pno in descending order, never
@@ -353,8 +378,11 @@ def pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total, pairs, t_cut
     block re-diagonalised, the energy with the semicanonical denominators).
     Returns (e_os, e_ss, kept, discarded, kept_occupation) arrays over pairs
     (the last, the sum of the kept occupations, is for the trace identity of
-    the tests); numpy eigh."""
-    b = _f64(b_ov)
+    the tests); numpy eigh. With ``g`` (a dense (nocc nvir)^2 matrix) each
+    pair's block is g[i nvir:(i+1) nvir, j nvir:(j+1) nvir] instead of
+    B_i^T B_j (the dense-g path, dlpno_mp2.rs:354-440); b_ov is then unused."""
+    b = _f64(b_ov) if g is None else None
+    gg = None if g is None else _f64(g)
     e = _f64(eps)
     pairs = [tuple(int(x) for x in p) for p in pairs]
     n = len(pairs)
@@ -362,7 +390,10 @@ def pno_pair_energies(b_ov, eps, nocc, nvir, first_occ, nocc_total, pairs, t_cut
     kept, disc, kocc = np.zeros(n, dtype=np.int64), np.zeros(n), np.zeros(n)
     ev = e[nocc_total:nocc_total + nvir]
     for k, (i, j) in enumerate(pairs):
-        g = b[:, i * nvir:(i + 1) * nvir].T @ b[:, j * nvir:(j + 1) * nvir]
+        if gg is None:
+            g = b[:, i * nvir:(i + 1) * nvir].T @ b[:, j * nvir:(j + 1) * nvir]
+        else:
+            g = gg[i * nvir:(i + 1) * nvir, j * nvir:(j + 1) * nvir]
         eij = e[first_occ + i] + e[first_occ + j]
         t = g / (eij - ev[:, None] - ev[None, :])
         d = t @ t.T + t.T @ t

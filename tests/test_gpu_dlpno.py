@@ -355,3 +355,44 @@ def test_domain_fit_refusals():
         domain_fit_pair_blocks(_cuda(a), _cuda(bad), xyz, cen, 3, 4, math.inf, [(0, 1)])
     with pytest.raises(LayoutError):
         domain_fit_pair_blocks(_cuda(a).float(), _cuda(v), xyz, cen, 3, 4, 1.0)
+
+
+# -- the dense-g entries (pair_energy_screen.rs:125-190, dlpno_mp2.rs:354-440) ------
+
+def test_dense_g_pair_energies_and_pno(oracle_mod):
+    from ferric_amd.dlpno import (dlpno_mp2_spin_components, estimate_pair_energies,
+                                  pno_pair_energies_from_g)
+
+    p, centers = synthetic.local_chain(nocc=8, nvir=40, rows_per_site=10)
+    nocc, nvir = p.nocc, p.nvir
+    rng = np.random.default_rng(77)
+    c = rng.standard_normal(p.b_ov.shape) * 0.02
+    for name, g in (("gram", p.b_ov.T @ p.b_ov), ("non-gram", p.b_ov.T @ (p.b_ov + c))):
+        print("dense g", name, flush=True)
+        gd = _cuda(g)
+        pe = estimate_pair_energies(gd, p.eps, nocc, nvir)
+        w_os, w_ss = oracle_mod.pair_energies_from_g(g, p.eps, nocc, nvir)
+        _close(pe.e_os, w_os, w_os)
+        _close(pe.e_ss, w_ss, w_os)
+        pairs = _pairs(nocc)
+        for t in (0.0, 1e-6):
+            g_os, g_ss, kept, worst = pno_pair_energies_from_g(gd, p.eps, nocc, nvir, 0, nocc,
+                                                               pairs, t)
+            v_os, v_ss, v_kept, v_disc, _ = oracle_mod.pno_pair_energies(
+                None, p.eps, nocc, nvir, 0, nocc, pairs, t, g=g)
+            assert kept == int(v_kept.sum())
+            assert np.max(np.abs(g_os - v_os)) <= 1e-10 * np.max(np.abs(v_os))
+            assert np.max(np.abs(g_ss - v_ss)) <= 1e-10 * np.max(np.abs(v_os))
+        d = build_pair_domains_by_energy(centers, pe, 1e-5)
+        sc, diag = dlpno_mp2_spin_components(gd, p.eps, nocc, nvir, 0, nocc, d,
+                                             DlpnoConfig(t_cut_pno=1e-6))
+        v_os, v_ss, v_kept, _, _ = oracle_mod.pno_pair_energies(None, p.eps, nocc, nvir, 0, nocc,
+                                                                d.pairs, 1e-6, g=g)
+        assert abs(sc.e_total - (np.sum(v_os) + np.sum(v_ss))) <= 1e-10 * abs(sc.e_total)
+        assert diag.virtual_retention == v_kept.sum() / (len(d.pairs) * nvir)
+        sc0, _ = dlpno_mp2_spin_components(gd, p.eps, nocc, nvir, 0, nocc, d)
+        assert abs(sc0.e_total - sum(pe.e_ij(i, j) for i, j in d.pairs)) <= 1e-12 * abs(sc0.e_total)
+    # the Gram case against the B entry
+    pb = estimate_pair_energies_from_b_ov(_cuda(p.b_ov), p.eps, nocc, nvir)
+    pg = estimate_pair_energies(_cuda(p.b_ov.T @ p.b_ov), p.eps, nocc, nvir)
+    _close(pg.e, pb.e, pb.e_os)

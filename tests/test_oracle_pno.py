@@ -109,3 +109,30 @@ def test_reference_properties(oracle_mod):
     assert np.array_equal(k2, kept)
     np.testing.assert_allclose(o2, os_, rtol=1e-12)
     np.testing.assert_allclose(s2, ss, rtol=1e-11, atol=1e-14 * np.max(np.abs(os_)))
+
+
+def test_dense_g_source_matches_the_b_source(oracle_mod):
+    """The dense-g restatements against the pinned B ones: with g = B^T B the
+    block source changes nothing; with a non-Gram g the pairs sum to the pinned
+    spin_components_from_g."""
+    b, eps = toy_b_ov(14, 4, 6)
+    nocc, nvir = 4, 6
+    pairs = _all_pairs(nocc)
+    g = b.T @ b
+    os0, ss0 = oracle_mod.pair_energies(b, eps, nocc, nvir)
+    os1, ss1 = oracle_mod.pair_energies_from_g(g, eps, nocc, nvir)
+    np.testing.assert_allclose(os1, os0, rtol=1e-12)
+    np.testing.assert_allclose(ss1, ss0, rtol=1e-12, atol=1e-13 * np.max(np.abs(os0)))
+    for t in (0.0, 1e-2):
+        w = oracle_mod.pno_pair_energies(b, eps, nocc, nvir, 0, nocc, pairs, t)
+        v = oracle_mod.pno_pair_energies(None, eps, nocc, nvir, 0, nocc, pairs, t, g=g)
+        assert np.array_equal(w[2], v[2])
+        np.testing.assert_allclose(v[0], w[0], rtol=1e-10)
+    rng = np.random.default_rng(9)
+    c = rng.uniform(-1, 1, b.shape)
+    gn = b.T @ c + 0.3 * c.T @ b
+    os2, ss2 = oracle_mod.pair_energies_from_g(gn, eps, nocc, nvir)
+    tot = oracle_mod.spin_components_from_g(gn, eps, nocc, nvir)
+    iu = np.triu_indices(nocc)
+    assert abs(np.sum(os2[iu]) - tot[0]) <= 1e-12 * abs(tot[0])
+    assert abs(np.sum(ss2[iu]) - tot[1]) <= 1e-12 * abs(tot[0])
