@@ -24,7 +24,9 @@ Method (this file's own code):
     and (P|Q), the inputs of the dressing / MO-transform stage
     (stream_dressed_mo_band, rimp2.rs:648-756).
 
-Two geometries: the equilibrium water of testdata/molecules/water.xyz (known
+Three molecules. H2 at 0.74 Angstrom, the reference's inline known-answer test
+(rimp2.rs:3187-3201: RI-MP2 within 1e-4 of the canonical MP2 -0.0263715576).
+Two water geometries: the equilibrium water of testdata/molecules/water.xyz (known
 answer testdata/reference/h2o_cc-pvdz_rimp2.json) and the distorted water of
 testdata/molecules/validation/h2o_distorted.xyz, whose ORCA 6.1.1 and PySCF
 2.13.1 results agree to 5e-11 Eh (testdata/reference/validation/
@@ -341,9 +343,10 @@ def to_sph(block, shells_axes):
     return out.reshape(shape)
 
 
-def build(xyz_path):
-    """RHF + RI factor at one geometry: dict of B, eps and the energies."""
-    lines = open(xyz_path).read().split("\n")[2:]
+def build(xyz_path, text=None):
+    """RHF + RI factor at one geometry (an .xyz file, or its text): dict of B,
+    eps and the energies."""
+    lines = (text if text is not None else open(xyz_path).read()).split("\n")[2:]
     atoms = []
     for ln in lines:
         f = ln.split()
@@ -512,6 +515,18 @@ def main():
                dist_e_rhf_ref=grad["pyscf"]["e_rhf"],
                # inputs of the dressing / MO transform: (Q|mu nu), MO coefficients, (P|Q)
                dist_j3=d["j3"], dist_c=d["c"], dist_j2=d["j2"])
+    # H2 / cc-pVDZ at 0.74 Angstrom, the reference's own known-answer test
+    # (rimp2.rs:3187-3201): RI-MP2 within 1e-4 of PySCF's canonical MP2
+    # -0.0263715576 (the RI error is of that order).
+    h = build(None, text="2\nH2\nH 0 0 0\nH 0 0 0.74\n")
+    os_, ss_ = numpy_mp2(h["b_ov"], h["eps"], h["nocc"], h["nvir"])
+    print(f"H2: RHF {h['e_rhf']:.10f}, RI-MP2 E_corr {os_ + ss_:.10f} (canonical MP2 "
+          f"-0.0263715576, diff {os_ + ss_ + 0.0263715576:.2e})", flush=True)
+    if abs(os_ + ss_ + 0.0263715576) > 1e-4:
+        sys.exit("H2 RI-MP2 energy is not within the reference test's 1e-4 of canonical MP2")
+    out.update(h2_b_ov=h["b_ov"], h2_eps=h["eps"], h2_nocc=h["nocc"], h2_nvir=h["nvir"],
+               h2_e_rhf=h["e_rhf"], h2_e_corr_numpy=os_ + ss_, h2_e_os_numpy=os_,
+               h2_e_corr_canonical=-0.0263715576)
     np.savez_compressed(OUT, **out)
     print("wrote", OUT)
 

@@ -188,7 +188,8 @@ def test_errors_are_typed(decane):
 def test_real_water_against_published_energies(oracle_mod):
     """cc-pVDZ / cc-pVDZ-RI water (tests/golden/water_ccpvdz_real.npz): the
     distorted geometry against ORCA/PySCF (1e-10 Eh, all-electron and one frozen
-    core), the equilibrium geometry against the reference's known answer (1e-7)."""
+    core), the equilibrium geometry against the reference's known answer (1e-7),
+    H2 against the reference's H2 known answer (1e-4) in FP64 and INT8X4."""
     import os
 
     w = np.load(os.path.join(os.path.dirname(__file__), "golden", "water_ccpvdz_real.npz"))
@@ -206,3 +207,11 @@ def test_real_water_against_published_energies(oracle_mod):
     assert abs(eq.e_total - float(w["e_corr_reference"])) <= 1e-7
     want = oracle_mod.spin_components(w["b_ov"], w["eps"], 5, 19)
     assert abs(eq.e_total - want[2]) <= 1e-12
+    # H2 / cc-pVDZ (rimp2.rs:3187-3201): within 1e-4 of canonical MP2, and the oracle
+    nh, vh = int(w["h2_nocc"]), int(w["h2_nvir"])
+    for prec in (Precision.FP64, Precision.INT8X4):
+        h2 = rimp2.spin_components(torch.from_numpy(w["h2_b_ov"]).cuda(), w["h2_eps"], nh, vh,
+                                   precision=prec)
+        assert abs(h2.e_total - float(w["h2_e_corr_canonical"])) < 1e-4
+        tol = 1e-12 if prec == Precision.FP64 else 1e-10  # INT8X4: 2^-32 per element
+        assert abs(h2.e_total - float(w["h2_e_corr_numpy"])) <= tol

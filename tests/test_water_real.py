@@ -52,3 +52,15 @@ def test_equilibrium_water_known_answer(water):
     e = oracle.spin_components(water["b_ov"], water["eps"], 5, 19)[2]
     assert abs(e - float(water["e_corr_reference"])) <= 1e-7
     assert e == pytest.approx(float(water["e_corr_numpy"]), abs=1e-13)
+
+
+def test_h2_known_answer(water):
+    """H2 / cc-pVDZ at 0.74 Angstrom, the reference's test_rimp2_h2_ccpvdz
+    (rimp2.rs:3187-3201): RI-MP2 within 1e-4 of PySCF's canonical MP2."""
+    nocc, nvir = int(water["h2_nocc"]), int(water["h2_nvir"])
+    e = oracle.spin_components(water["h2_b_ov"], water["h2_eps"], nocc, nvir)[2]
+    assert abs(e - float(water["h2_e_corr_canonical"])) < 1e-4
+    assert e == pytest.approx(float(water["h2_e_corr_numpy"]), abs=1e-14)
+    # one occupied orbital: only the (0, 0) pair, whose same-spin part is zero
+    e_os, e_ss, _ = oracle.spin_components(water["h2_b_ov"], water["h2_eps"], nocc, nvir)
+    assert nocc == 1 and abs(e_ss) <= 1e-15 * abs(e_os)
