@@ -493,6 +493,14 @@ def main():
                                                                           ch.nvir), 10)
         loc["pair_energies_fp64"] = {"ms_median": med, "ms_min": mn,
                                      "sum_minus_E_total": pe.total() - dense.e_total}
+        # the screen's estimates in the FP64-accurate INT8 digit-slice mode
+        pe8 = dlpno.estimate_pair_energies_from_b_ov(cb_, ch.eps, ch.nocc, ch.nvir,
+                                                     precision=Precision.INT8X4)
+        med, mn = timed_ms(lambda: dlpno.estimate_pair_energies_from_b_ov(
+            cb_, ch.eps, ch.nocc, ch.nvir, precision=Precision.INT8X4), 10)
+        loc["pair_energies_int8x4"] = {"ms_median": med, "ms_min": mn,
+                                       "max_abs_dE_pair": float(np.max(np.abs(pe8.e - pe.e))),
+                                       "sum_minus_fp64": pe8.total() - pe.total()}
         cfg = dlpno.DlpnoConfig()
 
         def default_dlpno():
@@ -528,7 +536,8 @@ def main():
         small["local MP2"] = loc
         local_chain = (ch, list(dom.pairs))
         log(f"local MP2 ({ch.name}): pair energies {loc['pair_energies_fp64']['ms_median']:.2f} "
-            f"ms, default DLPNO {loc['dlpno_default']['ms_median']:.2f} ms ({len(dom.pairs)} "
+            f"ms (INT8X4 {loc['pair_energies_int8x4']['ms_median']:.2f} ms, max |dE_pair| "
+            f"{loc['pair_energies_int8x4']['max_abs_dE_pair']:.1e}), default DLPNO {loc['dlpno_default']['ms_median']:.2f} ms ({len(dom.pairs)} "
             f"pairs, dE {loc['dlpno_default']['dE_vs_dense']:+.2e}), PNO 1e-6 "
             f"{loc['pno_1e-6']['ms_median']:.1f} ms (retention "
             f"{loc['pno_1e-6']['virtual_retention']:.3f}, dE "
