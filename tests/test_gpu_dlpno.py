@@ -236,12 +236,15 @@ def test_pno_truncation_against_oracle_local_chain(oracle_mod, t_cut):
     _pno_check(oracle_mod, p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, _pairs(p.nocc), t_cut)
 
 
-@pytest.mark.parametrize("t_cut", [1e-8, 1e-6])
-def test_pno_width_classes(oracle_mod, t_cut):
+@pytest.mark.parametrize("nocc,nvir,rows,seed,t_cut", [(6, 300, 12, 1010, 1e-8),
+                                                       (6, 300, 12, 1010, 1e-6),
+                                                       (5, 130, 40, 1011, 1e-8)])
+def test_pno_width_classes(oracle_mod, nocc, nvir, rows, seed, t_cut):
     """nvir 300: kept counts 1..192 put the pairs of one batch into the 128- and
-    256-wide classes of the semicanonical step."""
-    p, _ = synthetic.local_chain(nocc=6, nvir=300, rows_per_site=12, seed=1010)
-    print("pno width classes 6/300", t_cut, flush=True)
+    256-wide classes of the semicanonical step; nvir 130: kept counts 82..130
+    straddle the 128-wide class and the full width."""
+    p, _ = synthetic.local_chain(nocc=nocc, nvir=nvir, rows_per_site=rows, seed=seed)
+    print(f"pno width classes {nocc}/{nvir}", t_cut, flush=True)
     _pno_check(oracle_mod, p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, _pairs(p.nocc), t_cut)
 
 
