@@ -239,13 +239,21 @@ def test_pno_truncation_against_oracle_local_chain(oracle_mod, t_cut):
 @pytest.mark.parametrize("nocc,nvir,rows,seed,t_cut", [(6, 300, 12, 1010, 1e-8),
                                                        (6, 300, 12, 1010, 1e-6),
                                                        (5, 130, 40, 1011, 1e-8)])
-def test_pno_width_classes(oracle_mod, nocc, nvir, rows, seed, t_cut):
+def test_pno_width_classes(oracle_mod, monkeypatch, nocc, nvir, rows, seed, t_cut):
     """nvir 300: kept counts 1..192 put the pairs of one batch into the 128- and
     256-wide classes of the semicanonical step; nvir 130: kept counts 82..130
-    straddle the 128-wide class and the full width."""
+    straddle the 128-wide class and the full width. Batches of 4 pairs (classes
+    formed per batch) give the same pair energies."""
     p, _ = synthetic.local_chain(nocc=nocc, nvir=nvir, rows_per_site=rows, seed=seed)
     print(f"pno width classes {nocc}/{nvir}", t_cut, flush=True)
-    _pno_check(oracle_mod, p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, _pairs(p.nocc), t_cut)
+    one = _pno_check(oracle_mod, p.b_ov, p.eps, p.nocc, p.nvir, 0, p.nocc, _pairs(p.nocc), t_cut)
+    monkeypatch.setenv("FERRIC_PNO_BATCH", "4")
+    few = pno_pair_energies(_cuda(p.b_ov), p.eps, p.nocc, p.nvir, 0, p.nocc, _pairs(p.nocc),
+                            t_cut)
+    assert few[2] == one[2]
+    scale = np.max(np.abs(one[0]))
+    assert np.max(np.abs(few[0] - one[0])) <= 1e-12 * scale
+    assert np.max(np.abs(few[1] - one[1])) <= 1e-12 * scale
 
 
 def test_pno_reference_toy_and_frozen_core(oracle_mod, golden):
