@@ -458,6 +458,8 @@ def main():
                 ("same_spin_alpha", lambda: un.u_same_spin(ca), 21 * 22 * 229 ** 2 * 960),
                 ("same_spin_beta", lambda: un.u_same_spin(cb), 20 * 21 * 230 ** 2 * 960),
                 ("opposite_spin", lambda: un.u_opposite_spin(ca, cb),
+                 2 * 21 * 20 * 229 * 230 * 960),
+                ("opposite_spin_int8x4", lambda: un.u_opposite_spin(ca, cb, Precision.INT8X4),
                  2 * 21 * 20 * 229 * 230 * 960)):
             val = fn()
             med, mn = timed_ms(fn, 10)

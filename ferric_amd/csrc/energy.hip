@@ -219,6 +219,9 @@ hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st) {
         else if (a.tile == 64) e = launch_items(pair_rect_kernel<float, 64, 16>, a, st);
         else if (a.tile == 32) e = launch_items(pair_rect_kernel<float, 32, 16>, a, st);
         break;
+    case Prec::I8X4:
+        e = launch_energy_i8x4_rect(a, st);
+        break;
     default:
         break;
     }

@@ -349,7 +349,172 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
     pair::block_store(os, ss, w, a.item_partials + 4 * it.id);
 }
 
+// Opposite-spin rectangle (u_rimp2.rs:731-760) on the INT8 digit slices: item
+// (i, J, tA, tB) over every alpha i, beta J and tile of each side, one product
+// C = B_alpha(i, A)^T B_beta(J, B). Waves 0-3 take the even Q panels, waves 4-7
+// the odd ones; their int32 level sums merge exactly through LDS, and waves 0-3
+// add sum (ia|JB)^2 / D over the tile with weight 1. Operand format, fetch and
+// level combination as energy_i8x4_kernel; op0 / rexp hold the alpha rows
+// (nvir_pad), op1 / rexp_b the beta rows (nvir_pad_b), each buffer with TILE
+// spare rows after its last.
+template <int TILE>
+__global__ __launch_bounds__(512) void energy_i8x4_rect_kernel(EnergyArgs a) {
+    constexpr int TW = TILE / 2, NB = TW / 16;
+    constexpr int FB = 4096;
+    __shared__ __attribute__((aligned(16))) int xbuf[kLevels * TILE * TILE];
+    __shared__ int rexp_t[2][TILE];
+
+    pair::Item it;
+    it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
+    const int64_t ntp = (int64_t)a.ntile * a.ntile_b;
+    const int64_t pij = it.id / ntp;
+    const int t = (int)(it.id % ntp);
+    it.i = (int)(pij / a.nocc_b);
+    it.j = (int)(pij % a.nocc_b);
+    it.ta = t / a.ntile_b;
+    it.tb = t % a.ntile_b;
+
+    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
+    const int np = a.naux_pad / 64, nga = a.nvir_pad / 16, ngb = a.nvir_pad_b / 16;
+    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+    const int prod = wid >> 2, quad = wid & 3;
+    const pair::Lane L{A0, B0, quad >> 1, quad & 1, lane % 16, lane / 16};
+    const int8_t *pa[NB], *pb[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        pa[k] = (const int8_t *)a.op0 +
+                (int64_t)(it.i * nga + (A0 + L.wr * TW) / 16 + k) * np * FB + lane * 16;
+        pb[k] = (const int8_t *)a.op1 +
+                (int64_t)(it.j * ngb + (B0 + L.wc * TW) / 16 + k) * np * FB + lane * 16;
+    }
+    if (tid < TILE) {
+        const int v = A0 + tid;
+        rexp_t[0][tid] = v < a.nvir_pad ? a.rexp[(int64_t)it.i * a.nvir_pad + v] : 0;
+    } else if (tid < 2 * TILE) {
+        const int c = tid - TILE, v = B0 + c;
+        rexp_t[1][c] = v < a.nvir_pad_b ? a.rexp_b[(int64_t)it.j * a.nvir_pad_b + v] : 0;
+    }
+
+    bool rv[NB], cv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
+        cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad_b;
+    }
+    bool full = true;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
+    v4i acc[kLevels][NB][NB];
+#pragma unroll
+    for (int l = 0; l < kLevels; ++l)
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) acc[l][x][y] = (v4i){0, 0, 0, 0};
+
+    v4i fa[2][4][NB], fb[2][4][NB];
+    auto fetch = [&](int P, v4i(&ra)[4][NB], v4i(&rb)[4][NB]) {
+        const int64_t off = (int64_t)P * FB;
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                ra[d][k] = *(const v4i *)(pa[k] + off + d * 1024);
+                rb[d][k] = *(const v4i *)(pb[k] + off + d * 1024);
+            }
+    };
+    auto mma = [&](const v4i(&ra)[4][NB], const v4i(&rb)[4][NB], auto fullc) {
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y)
+                if (decltype(fullc)::value || (rv[x] && cv[y])) {
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (p + q < kLevels)
+                                acc[p + q][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                    ra[p][x], rb[q][y], acc[p + q][x][y], 0, 0, 0);
+                }
+    };
+    auto compute = [&](const v4i(&ra)[4][NB], const v4i(&rb)[4][NB]) {
+        if (full)
+            mma(ra, rb, std::true_type{});
+        else
+            mma(ra, rb, std::false_type{});
+    };
+    // Iteration p covers panel 2 p + prod.
+    const int iters = (np + 1) / 2;
+    auto valid = [&](int p) { return p < iters && 2 * p + prod < np; };
+    if (valid(0)) fetch(prod, fa[0], fb[0]);
+    for (int p = 0; p < iters; p += 2) {
+        if (valid(p + 1)) fetch(2 * (p + 1) + prod, fa[1], fb[1]);
+        if (valid(p)) compute(fa[0], fb[0]);
+        if (valid(p + 2)) fetch(2 * (p + 2) + prod, fa[0], fb[0]);
+        if (valid(p + 1)) compute(fa[1], fb[1]);
+    }
+    __syncthreads();  // rexp_t written; every wave done with its loop
+    if (prod == 1) {
+#pragma unroll
+        for (int l = 0; l < kLevels; ++l)
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                        const int cb = L.wc * TW + y * 16 + L.r;
+                        xbuf[(l * TILE + ra) * TILE + cb] = acc[l][x][y][v];
+                    }
+    }
+    __syncthreads();
+    const double eij = a.eps_occ[it.i] + a.eps_occ_b[it.j];
+    KSum os, ss;
+    if (prod == 0) {
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) {
+                if (!(rv[x] && cv[y])) continue;
+                const int cb = L.wc * TW + y * 16 + L.r;
+                if (B0 + cb >= a.nvir_b) continue;
+                const double eb = a.eps_vir_b[B0 + cb];
+                const int xb = rexp_t[1][cb];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
+                    if (A0 + ra >= a.nvir) continue;
+                    int sl[kLevels];
+#pragma unroll
+                    for (int l = 0; l < kLevels; ++l)
+                        sl[l] = acc[l][x][y][v] + xbuf[(l * TILE + ra) * TILE + cb];
+                    const double g = combine_levels(sl[0], sl[1], sl[2], sl[3], sl[4],
+                                                    rexp_t[0][ra] + xb);
+                    os.add((g * g) / (eij - a.eps_vir[A0 + ra] - eb));
+                }
+            }
+    }
+    pair::block_store(os, ss, 1.0, a.item_partials + 4 * it.id);
+}
+
 }  // namespace
+
+hipError_t launch_energy_i8x4_rect(const EnergyArgs &a, hipStream_t st) {
+    if (!a.rexp || !a.rexp_b || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad ||
+        a.nvir_pad % 16 != 0 || a.nvir_pad_b % 16 != 0)
+        return hipErrorInvalidValue;
+    if (a.tile == 64)
+        hipLaunchKernelGGL((energy_i8x4_rect_kernel<64>), dim3((unsigned)a.nitems), dim3(512), 0,
+                           st, a);
+    else if (a.tile == 32)
+        hipLaunchKernelGGL((energy_i8x4_rect_kernel<32>), dim3((unsigned)a.nitems), dim3(512), 0,
+                           st, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st) {
     if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad || a.nvir_pad % 16 != 0)

@@ -44,6 +44,7 @@ enum SlotId {
     kSlotFitWork,    // domain fits: padded V_DD and [A_i | A_j] per batched pair
     kSlotFitMeta,    // domain fits: domain index lists, sizes, pairs, Cholesky info
     kSlotGPairs,     // energies from explicit (ia|jb) blocks: the pair list
+    kSlotRexpB,      // INT8X4 opposite spin: row exponents of the beta rows
     kNumSlots
 };
 

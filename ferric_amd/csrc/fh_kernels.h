@@ -143,6 +143,7 @@ struct EnergyArgs {
     // alpha i, beta J and tile of each side.
     int nocc_b = 0, nvir_b = 0, nvir_pad_b = 0, ntile_b = 0;
     const double *eps_occ_b = nullptr, *eps_vir_b = nullptr;
+    const int *rexp_b = nullptr;  // I8X4: row exponents of the beta rows
 };
 
 hipError_t launch_energy(const EnergyArgs &a, hipStream_t st);
@@ -171,6 +172,7 @@ struct GemmArgs {
 hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st);
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st);
+hipError_t launch_energy_i8x4_rect(const EnergyArgs &a, hipStream_t st);
 // Phase A of the adjoint on the INT8 digit-slice kernel, every item in one
 // launch (i_fixed < 0), T~ into the whole-T~ matrix.
 hipError_t launch_adjoint_i8x4(const EnergyArgs &a, hipStream_t st);

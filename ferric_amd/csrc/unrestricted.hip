@@ -49,18 +49,31 @@ fh_status opposite_core(State &s, const fh_spin_channel &ca, const fh_spin_chann
     const int64_t nitems = (int64_t)ca.nocc * (int64_t)cb.nocc * ntile_a * ntile_b;
     if (nitems > 0x7fffffffLL) return fail(FH_UNSUPPORTED, "%lld work items", (long long)nitems);
     const size_t eb = packed_elem_bytes(prec);
-    const size_t bytes_a = ca.nocc * (size_t)vpa * naux_pad * eb;
-    const size_t bytes_b = cb.nocc * (size_t)vpb * naux_pad * eb;
+    // INT8X4 operands carry TILE spare rows after their last (the fetch of a
+    // narrow last tile reads whole 16-row blocks)
+    const bool i8 = prec == Prec::I8X4;
+    const size_t rows_a = ca.nocc * (size_t)vpa + (i8 ? (size_t)tile : 0);
+    const size_t rows_b = cb.nocc * (size_t)vpb + (i8 ? (size_t)tile : 0);
+    const size_t bytes_a = rows_a * naux_pad * eb;
+    const size_t bytes_b = rows_b * naux_pad * eb;
     const size_t neps = ca.nocc + ca.nvir + cb.nocc + cb.nvir;
-    fh_status st = reserve(s, {{kSlotOp0, bytes_a},
-                               {kSlotOp1, bytes_b},
-                               {kSlotEps, sizeof(double) * neps},
-                               {kSlotItems, sizeof(double) * 4 * (size_t)nitems},
-                               {kSlotRows, sizeof(double) * 4 * ca.nocc},
-                               {kSlotResult, sizeof(double) * (2 + 2 * ca.nocc)},
-                               {kSlotScalar, 256}});
+    std::vector<std::pair<int, size_t>> req = {{kSlotOp0, bytes_a},
+                                               {kSlotOp1, bytes_b},
+                                               {kSlotEps, sizeof(double) * neps},
+                                               {kSlotItems, sizeof(double) * 4 * (size_t)nitems},
+                                               {kSlotRows, sizeof(double) * 4 * ca.nocc},
+                                               {kSlotResult, sizeof(double) * (2 + 2 * ca.nocc)},
+                                               {kSlotScalar, 256}};
+    if (i8) {
+        if (naux_pad > kI8MaxNauxPad)
+            return fail(FH_RANGE, "naux_pad %d beyond the INT8 slices' int32 range (%d)",
+                        naux_pad, kI8MaxNauxPad);
+        req.push_back({kSlotScale, sizeof(int) * rows_a});
+        req.push_back({kSlotRexpB, sizeof(int) * rows_b});
+    }
+    fh_status st = reserve(s, req);
     if (st != FH_OK) return st;
-    if (prec != Prec::F64) {
+    if (prec != Prec::F64 && !i8) {
         st = range_check(s, ba, in_f32, naux * ca.nocc * ca.nvir, naux, stream);
         if (st == FH_OK) st = range_check(s, bb, in_f32, naux * cb.nocc * cb.nvir, naux, stream);
         if (st != FH_OK) return st;
@@ -82,16 +95,19 @@ fh_status opposite_core(State &s, const fh_spin_channel &ca, const fh_spin_chann
     pk.in_f32 = in_f32;
     pk.naux = (int)naux;
     pk.naux_pad = naux_pad;
-    pk.fmt = prec == Prec::F64 ? PackFmt::F64 : PackFmt::F32;
+    pk.fmt = prec == Prec::F64 ? PackFmt::F64 : i8 ? PackFmt::I8X4 : PackFmt::F32;
     pk.out1 = nullptr;
     pk.scale = nullptr;
     pk.lo_sq = nullptr;
+    int *rexp_a = i8 ? (int *)s.slots[kSlotScale].ptr : nullptr;
+    int *rexp_b = i8 ? (int *)s.slots[kSlotRexpB].ptr : nullptr;
     mark(s, 0, stream);
     pk.b = ba;
     pk.nocc = (int)ca.nocc;
     pk.nvir = (int)ca.nvir;
     pk.nvir_pad = vpa;
     pk.out0 = s.slots[kSlotOp0].ptr;
+    pk.rexp = rexp_a;
     hipError_t he = launch_pack(pk, stream);
     if (he == hipSuccess) {
         pk.b = bb;
@@ -99,6 +115,7 @@ fh_status opposite_core(State &s, const fh_spin_channel &ca, const fh_spin_chann
         pk.nvir = (int)cb.nvir;
         pk.nvir_pad = vpb;
         pk.out0 = s.slots[kSlotOp1].ptr;
+        pk.rexp = rexp_b;
         he = launch_pack(pk, stream);
     }
     if (he != hipSuccess) {
@@ -133,6 +150,8 @@ fh_status opposite_core(State &s, const fh_spin_channel &ca, const fh_spin_chann
     ea.ntile_b = ntile_b;
     ea.eps_occ_b = deps + ca.nocc + ca.nvir;
     ea.eps_vir_b = deps + ca.nocc + ca.nvir + cb.nocc;
+    ea.rexp = rexp_a;
+    ea.rexp_b = rexp_b;
     mark(s, 1, stream);
     he = launch_energy_rect(ea, stream);
     if (he != hipSuccess) {
@@ -257,8 +276,8 @@ static fh_status opposite_entry(const fh_spin_channel *alpha, const fh_spin_chan
     if (!energy) return fail(FH_LAYOUT, "null output pointer");
     Prec prec;
     if (!to_prec(precision, &prec)) return fail(FH_LAYOUT, "unknown precision %d", (int)precision);
-    if (prec != Prec::F64 && prec != Prec::F32)
-        return fail(FH_UNSUPPORTED, "the opposite-spin kernel runs in FP64 or FP32");
+    if (prec != Prec::F64 && prec != Prec::F32 && prec != Prec::I8X4)
+        return fail(FH_UNSUPPORTED, "the opposite-spin kernel runs in FP64, FP32 or INT8X4");
     if (b_dtype != FH_DT_F64 && b_dtype != FH_DT_F32)
         return fail(FH_LAYOUT, "unknown b_dtype %d", (int)b_dtype);
     bool ea = false, eb = false;

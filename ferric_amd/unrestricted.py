@@ -102,7 +102,7 @@ def u_same_spin(ch: SpinChannel, precision: Precision = Precision.FP64) -> float
 def u_opposite_spin(ch_a: SpinChannel, ch_b: SpinChannel,
                     precision: Precision = Precision.FP64) -> float:
     """Opposite-spin energy sum_{i,J,a,B} (ia|JB)^2 / D (u_rimp2.rs:731-760);
-    both channels fitted in one aux basis. FP64 or FP32."""
+    both channels fitted in one aux basis. FP64, FP32 or INT8X4."""
     a, b = _Chan(ch_a), _Chan(ch_b)
     if a.naux != b.naux:
         raise LayoutError(f"alpha has {a.naux} aux rows, beta {b.naux}: one aux basis is required")
@@ -144,12 +144,12 @@ def try_u_same_spin_on_device(ch: SpinChannel, mixed_ok: bool = False):
 
 def try_u_opposite_spin_on_device(ch_a: SpinChannel, ch_b: SpinChannel, mixed_ok: bool = False):
     """rimp2_gpu.rs:893-902: the energy, or None for the CPU path. The
-    opposite-spin block runs in FP64 (FP32 when mixed precision is allowed and
-    selected)."""
+    opposite-spin block runs in FP64 (FP32 or INT8X4 when mixed precision is
+    allowed and selected)."""
     if os.environ.get("FERRIC_GPU", "").lower() == "off":
         return None
     prec = _mixed_precision(mixed_ok)
-    if prec not in (Precision.FP64, Precision.FP32):
+    if prec not in (Precision.FP64, Precision.FP32, Precision.INT8X4):
         prec = Precision.FP64
     try:
         try:

@@ -407,7 +407,9 @@ fh_status fh_u_same_spin_dev(const fh_spin_channel *ch, fh_dtype b_dtype, size_t
 /* The drop-in for try_u_opposite_spin_on_device (rimp2_gpu.rs:893-902): the
  * opposite-spin energy sum_{i,J} sum_{a,B} (ia|JB)^2 / D of an alpha and a beta
  * channel fitted in one aux basis (opposite_spin_block_energy,
- * u_rimp2.rs:731-760). precision: FH_FP64 or FH_FP32. */
+ * u_rimp2.rs:731-760). precision: FH_FP64, FH_FP32 or FH_INT8X4 (the INT8X4
+ * mode, the other precisions are FH_UNSUPPORTED; naux beyond its int32 range
+ * is FH_RANGE). */
 fh_status fh_u_opposite_spin(const fh_spin_channel *alpha, const fh_spin_channel *beta,
                              size_t naux, fh_precision precision, double *energy);
 fh_status fh_u_opposite_spin_dev(const fh_spin_channel *alpha, const fh_spin_channel *beta,

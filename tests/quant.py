@@ -105,19 +105,42 @@ def int8x4_g(b):
     """The (nov, nov) matrix g = (ia|jb) the INT8X4 kernel forms, bit for bit:
     level sums S_L (exact int64), t = S6 2^-32 + S5 2^-24 + S4 2^-16 + S3 2^-8 + S2
     summed in that order in FP64, then g = t 2^(e_ia + e_jb - 12)."""
-    e, d = int8x4_digits(b)
-    nov = d.shape[2]
-    s = [np.zeros((nov, nov), dtype=np.int64) for _ in range(5)]
+    return int8x4_g_rect(b, b)
+
+
+def int8x4_g_rect(ba, bb):
+    """int8x4_g for two factors (the opposite-spin rectangle): (nov_a, nov_b)
+    g[ia, JB] from the digits of each factor under its own row exponents."""
+    ea, da = int8x4_digits(ba)
+    eb, db = int8x4_digits(bb)
+    s = [np.zeros((da.shape[2], db.shape[2]), dtype=np.int64) for _ in range(5)]
     for p in range(4):
         for q in range(4):
             if p + q < 5:
-                s[p + q] += d[p].T @ d[q]
+                s[p + q] += da[p].T @ db[q]
     t = s[4].astype(np.float64) * 2.0 ** -32
     t = t + s[3].astype(np.float64) * 2.0 ** -24
     t = t + s[2].astype(np.float64) * 2.0 ** -16
     t = t + s[1].astype(np.float64) * 2.0 ** -8
     t = t + s[0].astype(np.float64)
-    return np.ldexp(t, e[:, None] + e[None, :] - 12)
+    return np.ldexp(t, ea[:, None] + eb[None, :] - 12)
+
+
+def int8x4_g_bound_rect(ba, bb):
+    """int8x4_g_bound for two factors: per-element bound on
+    |int8x4_g_rect(ba, bb) - ba^T bb|."""
+    x = np.asarray(ba, dtype=np.float64)
+    y = np.asarray(bb, dtype=np.float64)
+    naux = x.shape[0]
+    ex, _ = int8x4_digits(x)
+    ey, _ = int8x4_digits(y)
+    ux, uy = np.ldexp(1.0, ex - 31), np.ldexp(1.0, ey - 31)
+    sx, sy = np.abs(x).sum(axis=0), np.abs(y).sum(axis=0)
+    unit = np.ldexp(1.0, ex[:, None] + ey[None, :] - 60)
+    bnd = np.outer(ux, sy) + np.outer(sx, uy) + naux * np.outer(ux, uy)
+    bnd += naux * (2.0 * 128 ** 2 * 2 ** 8 + 128 ** 2) * unit
+    bnd += 6 * 2.0 ** -53 * (np.abs(x).T @ np.abs(y) + bnd)
+    return bnd
 
 
 def int8x4_g_bound(b):

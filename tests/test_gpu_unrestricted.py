@@ -153,5 +153,40 @@ def test_same_spin_int8x4_within_the_format_bound(pair_mid, oracle_mod):
                                            ch.nocc_total)
         assert abs(got - want) <= 0.5 * bss + 1e-14, (got, want, bss)
         assert abs(got - want) <= 1e-8
-    with pytest.raises(ferric_amd.UnsupportedError):
-        u.u_opposite_spin(pair_mid[0], pair_mid[1], Precision.INT8X4)
+
+
+def _rect_energy(g, ch_a, ch_b):
+    """sum (ia|JB)^2 / D over every alpha i, a and beta J, B of g (nov_a, nov_b)."""
+    eo_a = ch_a.eps[ch_a.first_occ:ch_a.first_occ + ch_a.nocc]
+    ev_a = ch_a.eps[ch_a.nocc_total:ch_a.nocc_total + ch_a.nvir]
+    eo_b = ch_b.eps[ch_b.first_occ:ch_b.first_occ + ch_b.nocc]
+    ev_b = ch_b.eps[ch_b.nocc_total:ch_b.nocc_total + ch_b.nvir]
+    d = (eo_a[:, None, None, None] + eo_b[None, None, :, None] - ev_a[None, :, None, None]
+         - ev_b[None, None, None, :])
+    g4 = g.reshape(ch_a.nocc, ch_a.nvir, ch_b.nocc, ch_b.nvir)
+    return g4, d
+
+
+@pytest.mark.parametrize("tile", ["32", "64"])
+def test_opposite_spin_int8x4(pair_small, pair_mid, oracle_mod, tile, monkeypatch):
+    """The opposite-spin rectangle through INT8 digit slices: equal to the energy
+    of exactly the g the format forms (quant.int8x4_g_rect) to 1e-13, and within
+    the format's derived bound (and 1e-8 Eh) of the oracle (u_rimp2.rs:731-760)."""
+    from tests import quant
+
+    monkeypatch.setenv("FERRIC_HIP_TILE", tile)
+    for a, b in (pair_small, pair_mid):
+        print("uhf opposite-spin int8x4", tile, a.nocc, a.nvir, b.nocc, b.nvir, flush=True)
+        got = u.u_opposite_spin(a, b, Precision.INT8X4)
+        g4, d = _rect_energy(quant.int8x4_g_rect(a.b, b.b), a, b)
+        fmt = float(np.sum(g4 * g4 / d))
+        assert abs(got - fmt) <= 1e-13 * abs(fmt), (got, fmt)
+        ref = oracle_mod.u_opposite_spin(_oracle_ch(a), _oracle_ch(b))
+        G4, _ = _rect_energy(a.b.T @ b.b, a, b)
+        dl4, _ = _rect_energy(quant.int8x4_g_bound_rect(a.b, b.b), a, b)
+        bound = float(np.sum((2 * np.abs(G4) * dl4 + dl4 * dl4) / np.abs(d)))
+        print(f"  dE {got - ref:.2e} bound {bound:.2e}", flush=True)
+        assert abs(got - ref) <= bound + 1e-14 and abs(got - ref) <= 1e-8
+    da = u.SpinChannel(torch.from_numpy(a.b).cuda(), a.eps, a.nocc, a.nvir, a.first_occ)
+    db = u.SpinChannel(torch.from_numpy(b.b).cuda(), b.eps, b.nocc, b.nvir, b.first_occ)
+    assert u.u_opposite_spin(da, db, Precision.INT8X4) == got
