@@ -190,3 +190,21 @@ def test_opposite_spin_int8x4(pair_small, pair_mid, oracle_mod, tile, monkeypatc
     da = u.SpinChannel(torch.from_numpy(a.b).cuda(), a.eps, a.nocc, a.nvir, a.first_occ)
     db = u.SpinChannel(torch.from_numpy(b.b).cuda(), b.eps, b.nocc, b.nvir, b.first_occ)
     assert u.u_opposite_spin(da, db, Precision.INT8X4) == got
+
+
+def test_opposite_spin_int8x4_odd_shapes(oracle_mod):
+    """One Q panel (waves 4-7 idle), one beta occupied, virtual counts off the
+    16-row groups and across tiles: still the exact format's energy."""
+    from tests import quant
+
+    rng = np.random.default_rng(77)
+    for naux, (na, va), (nb, vb) in ((40, (3, 5), (1, 70)), (130, (2, 67), (3, 9))):
+        a = _channel(rng, naux, na, va)
+        b = _channel(rng, naux, nb, vb)
+        print("uhf opposite-spin int8x4 odd", naux, na, va, nb, vb, flush=True)
+        got = u.u_opposite_spin(a, b, Precision.INT8X4)
+        g4, d = _rect_energy(quant.int8x4_g_rect(a.b, b.b), a, b)
+        fmt = float(np.sum(g4 * g4 / d))
+        assert abs(got - fmt) <= 1e-13 * abs(fmt), (got, fmt)
+        ref = oracle_mod.u_opposite_spin(_oracle_ch(a), _oracle_ch(b))
+        assert abs(got - ref) <= 1e-8
