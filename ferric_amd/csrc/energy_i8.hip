@@ -39,7 +39,10 @@ namespace fh {
 namespace {
 
 constexpr int kLevels = 5;  // digit-product levels L = 2..6, index L - 2
-constexpr int kBand = 4;    // tile rows per band of the tile-pair order
+#ifndef FH_I8_BAND
+#define FH_I8_BAND 4
+#endif
+constexpr int kBand = FH_I8_BAND;  // tile rows per band of the tile-pair order
 
 // Workgroup b runs on XCD b % 8. Item of workgroup b: with the occupied pairs
 // in groups of eight, XCD x takes pair x of every group (pairs x, x + 8, ...)
