@@ -26,6 +26,9 @@ Method (this file's own code):
 
 Three molecules. H2 at 0.74 Angstrom, the reference's inline known-answer test
 (rimp2.rs:3187-3201: RI-MP2 within 1e-4 of the canonical MP2 -0.0263715576).
+The equilibrium water once more with a density-fitted SCF (J and K fitted in
+def2-universal-jkfit, the reference's scf_jk = "rijk"; known answer
+testdata/reference/h2o_cc-pvdz_rijk-def2-universal-jkfit_dfmp2.json, PySCF).
 Two water geometries: the equilibrium water of testdata/molecules/water.xyz (known
 answer testdata/reference/h2o_cc-pvdz_rimp2.json) and the distorted water of
 testdata/molecules/validation/h2o_distorted.xyz, whose ORCA 6.1.1 and PySCF
@@ -59,6 +62,8 @@ REF_GRAD = os.path.join(REF, "testdata", "reference", "validation", "rimp2_gradi
                         "h2o_distorted_cc-pvdz.json")
 REF_SCS = os.path.join(REF, "testdata", "reference", "validation", "scs_mp2_gradient",
                        "h2o_distorted_cc-pvdz.json")
+REF_RIJK = os.path.join(REF, "testdata", "reference",
+                        "h2o_cc-pvdz_rijk-def2-universal-jkfit_dfmp2.json")
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "water_ccpvdz_real.npz")
 BOHR_ANGSTROM = 0.52917721092   # CODATA 2010, the reference's Å <-> Bohr constant
 Z = {"H": 1, "O": 8}
@@ -89,7 +94,17 @@ def spherical_transform(l):
                 {(3, 0, 0): 1, (1, 2, 0): -3},
                 {(2, 1, 0): 3, (0, 3, 0): -1}]
     else:
-        raise ValueError(l)
+        # higher l: the harmonic polynomials of degree l, the null space of the
+        # Laplacian from degree l to degree l - 2 (any basis of it will do)
+        low = cart_components(l - 2)
+        lidx = {c: k for k, c in enumerate(low)}
+        lap = np.zeros((len(low), len(comps)))
+        for k, (a, b, c) in enumerate(comps):
+            for (n, t) in ((a, (a - 2, b, c)), (b, (a, b - 2, c)), (c, (a, b, c - 2))):
+                if n >= 2:
+                    lap[lidx[t], k] += n * (n - 1)
+        _, sv, vt = np.linalg.svd(lap)
+        return vt[len(low):]
     m = np.zeros((len(rows), len(comps)))
     for r, d in enumerate(rows):
         for c, v in d.items():
@@ -343,9 +358,55 @@ def to_sph(block, shells_axes):
     return out.reshape(shape)
 
 
-def build(xyz_path, text=None):
+def four_centre(ao, aoff, nao, pairs, nrm):
+    """The exact (mu nu|lambda sigma), normalised like the one-electron integrals."""
+    eri = np.zeros((nao,) * 4)
+    for i in range(len(ao)):
+        for j in range(i + 1):
+            for k in range(len(ao)):
+                for l in range(k + 1):
+                    if (i, j) < (k, l):
+                        continue
+                    raw = two_electron(pairs[(i, j)], pairs[(k, l)])   # (ca,cb,cc,cd,fa,fb,fc,fd)
+                    blk = to_sph(raw, [ao[i], ao[j], ao[k], ao[l]])
+                    si, sj, sk, sl_ = (np.s_[aoff[x]:aoff[x] + ao[x].nfunc] for x in (i, j, k, l))
+                    for (a_, b_, c_, d_, arr) in (
+                            (si, sj, sk, sl_, blk), (sj, si, sk, sl_, blk.transpose(1, 0, 2, 3)),
+                            (si, sj, sl_, sk, blk.transpose(0, 1, 3, 2)),
+                            (sj, si, sl_, sk, blk.transpose(1, 0, 3, 2))):
+                        eri[a_, b_, c_, d_] = arr
+                        eri[c_, d_, a_, b_] = arr.transpose(2, 3, 0, 1)
+    eri *= np.einsum("p,q,r,s->pqrs", nrm, nrm, nrm, nrm)
+    return eri
+
+
+def fit_integrals(aux, ao, pairs, nrm):
+    """(Q|mu nu) (naux, nao, nao) and (P|Q) of an aux basis."""
+    aoff, nao = offsets(ao)
+    xoff, naux = offsets(aux)
+    j3 = np.zeros((naux, nao, nao))
+    for x, sx in enumerate(aux):
+        px = Pair(sx)
+        for i in range(len(ao)):
+            for j in range(len(ao)):
+                raw = two_electron(px, pairs[(i, j)])   # (cx, 1, ca, cb, fx, 1, fa, fb)
+                raw = raw[:, 0][:, :, :, :, 0]           # (cx, ca, cb, fx, fa, fb)
+                j3[xoff[x]:xoff[x] + sx.nfunc, aoff[i]:aoff[i] + ao[i].nfunc,
+                   aoff[j]:aoff[j] + ao[j].nfunc] = to_sph(raw, [sx, ao[i], ao[j]])
+    j3 *= nrm[None, :, None] * nrm[None, None, :]
+    j2 = np.zeros((naux, naux))
+    for x, sx in enumerate(aux):
+        for y, sy in enumerate(aux):
+            raw = two_electron(Pair(sx), Pair(sy))[:, 0, :, 0][:, :, :, 0, :, 0]   # (cx, cy, fx, fy)
+            j2[xoff[x]:xoff[x] + sx.nfunc, xoff[y]:xoff[y] + sy.nfunc] = to_sph(raw, [sx, sy])
+    return j3, j2
+
+
+def build(xyz_path, text=None, jk_fit=None):
     """RHF + RI factor at one geometry (an .xyz file, or its text): dict of B,
-    eps and the energies."""
+    eps and the energies. jk_fit names a JK fitting basis: the SCF then uses
+    density-fitted (mu nu|lambda sigma) = sum_PQ (mu nu|P) (P|Q)^-1 (Q|lambda sigma)
+    (the reference's scf_jk = "rijk") instead of the exact four-centre integrals."""
     lines = (text if text is not None else open(xyz_path).read()).split("\n")[2:]
     atoms = []
     for ln in lines:
@@ -374,23 +435,12 @@ def build(xyz_path, text=None):
     S, T, V = (M * nrm[:, None] * nrm[None, :] for M in (S, T, V))
 
     pairs = {(i, j): Pair(ao[i], ao[j]) for i in range(len(ao)) for j in range(len(ao))}
-    eri = np.zeros((nao,) * 4)
-    for i in range(len(ao)):
-        for j in range(i + 1):
-            for k in range(len(ao)):
-                for l in range(k + 1):
-                    if (i, j) < (k, l):
-                        continue
-                    raw = two_electron(pairs[(i, j)], pairs[(k, l)])   # (ca,cb,cc,cd,fa,fb,fc,fd)
-                    blk = to_sph(raw, [ao[i], ao[j], ao[k], ao[l]])
-                    si, sj, sk, sl_ = (np.s_[aoff[x]:aoff[x] + ao[x].nfunc] for x in (i, j, k, l))
-                    for (a_, b_, c_, d_, arr) in (
-                            (si, sj, sk, sl_, blk), (sj, si, sk, sl_, blk.transpose(1, 0, 2, 3)),
-                            (si, sj, sl_, sk, blk.transpose(0, 1, 3, 2)),
-                            (sj, si, sl_, sk, blk.transpose(1, 0, 3, 2))):
-                        eri[a_, b_, c_, d_] = arr
-                        eri[c_, d_, a_, b_] = arr.transpose(2, 3, 0, 1)
-    eri *= np.einsum("p,q,r,s->pqrs", nrm, nrm, nrm, nrm)
+    if jk_fit is not None:
+        jk3, jk2 = fit_integrals(load_basis(jk_fit, atoms), ao, pairs, nrm)
+        xf = np.linalg.solve(np.linalg.cholesky(jk2), jk3.reshape(jk3.shape[0], -1))
+        eri = (xf.T @ xf).reshape((nao,) * 4)
+    else:
+        eri = four_centre(ao, aoff, nao, pairs, nrm)
 
     enuc = 0.0
     for a in range(len(atoms)):
@@ -438,21 +488,7 @@ def build(xyz_path, text=None):
     e_rhf = 0.5 * np.sum(D * (Hc + F)) + enuc
 
     # fit integrals
-    j3 = np.zeros((naux, nao, nao))
-    for x, sx in enumerate(aux):
-        px = Pair(sx)
-        for i in range(len(ao)):
-            for j in range(len(ao)):
-                raw = two_electron(px, pairs[(i, j)])   # (cx, 1, ca, cb, fx, 1, fa, fb)
-                raw = raw[:, 0][:, :, :, :, 0]           # (cx, ca, cb, fx, fa, fb)
-                j3[xoff[x]:xoff[x] + sx.nfunc, aoff[i]:aoff[i] + ao[i].nfunc,
-                   aoff[j]:aoff[j] + ao[j].nfunc] = to_sph(raw, [sx, ao[i], ao[j]])
-    j3 *= nrm[None, :, None] * nrm[None, None, :]
-    j2 = np.zeros((naux, naux))
-    for x, sx in enumerate(aux):
-        for y, sy in enumerate(aux):
-            raw = two_electron(Pair(sx), Pair(sy))[:, 0, :, 0][:, :, :, 0, :, 0]   # (cx, cy, fx, fy)
-            j2[xoff[x]:xoff[x] + sx.nfunc, xoff[y]:xoff[y] + sy.nfunc] = to_sph(raw, [sx, sy])
+    j3, j2 = fit_integrals(aux, ao, pairs, nrm)
     nvir = nao - nocc
     co, cv = C[:, :nocc], C[:, nocc:]
     pia = np.einsum("Pmn,mi,na->Pia", j3, co, cv).reshape(naux, nocc * nvir)
@@ -515,6 +551,21 @@ def main():
                dist_e_rhf_ref=grad["pyscf"]["e_rhf"],
                # inputs of the dressing / MO transform: (Q|mu nu), MO coefficients, (P|Q)
                dist_j3=d["j3"], dist_c=d["c"], dist_j2=d["j2"])
+    # Water with a density-fitted SCF (def2-universal-jkfit for J and K) and the
+    # cc-pVDZ-RI fit for MP2: the reference's RI-JK known answer
+    # (testdata/reference/h2o_cc-pvdz_rijk-def2-universal-jkfit_dfmp2.json, PySCF).
+    rj = json.load(open(REF_RIJK))
+    r = build(XYZ, jk_fit=rj["jk_auxbasis"])
+    os_, ss_ = numpy_mp2(r["b_ov"], r["eps"], r["nocc"], r["nvir"])
+    print(f"RI-JK water: RHF {r['e_rhf']:.12f} (PySCF {rj['rhf_energy']:.12f}, diff "
+          f"{r['e_rhf'] - rj['rhf_energy']:.2e}); E_corr {os_ + ss_:.12f} (PySCF "
+          f"{rj['mp2_corr']:.12f}, diff {os_ + ss_ - rj['mp2_corr']:.2e})", flush=True)
+    if abs(r["e_rhf"] - rj["rhf_energy"]) > 1e-8:
+        sys.exit("RI-JK RHF energy does not match the reference")
+    out.update(rijk_b_ov=r["b_ov"], rijk_eps=r["eps"], rijk_e_rhf=r["e_rhf"],
+               rijk_e_rhf_reference=rj["rhf_energy"], rijk_e_corr_numpy=os_ + ss_,
+               rijk_e_corr_reference=rj["mp2_corr"])
+
     # H2 / cc-pVDZ at 0.74 Angstrom, the reference's own known-answer test
     # (rimp2.rs:3187-3201): RI-MP2 within 1e-4 of PySCF's canonical MP2
     # -0.0263715576 (the RI error is of that order).

@@ -189,7 +189,8 @@ def test_real_water_against_published_energies(oracle_mod):
     """cc-pVDZ / cc-pVDZ-RI water (tests/golden/water_ccpvdz_real.npz): the
     distorted geometry against ORCA/PySCF (1e-10 Eh, all-electron and one frozen
     core), the equilibrium geometry against the reference's known answer (1e-7),
-    H2 against the reference's H2 known answer (1e-4) in FP64 and INT8X4."""
+    H2 against the reference's H2 known answer (1e-4) in FP64 and INT8X4, and
+    the RI-JK water against its PySCF known answer (1e-10 Eh)."""
     import os
 
     w = np.load(os.path.join(os.path.dirname(__file__), "golden", "water_ccpvdz_real.npz"))
@@ -207,6 +208,11 @@ def test_real_water_against_published_energies(oracle_mod):
     assert abs(eq.e_total - float(w["e_corr_reference"])) <= 1e-7
     want = oracle_mod.spin_components(w["b_ov"], w["eps"], 5, 19)
     assert abs(eq.e_total - want[2]) <= 1e-12
+    # RI-JK water (PySCF DF-RHF + DF-MP2): 1e-10 Eh in FP64, INT8X4 within 1e-8
+    for prec, tol in ((Precision.FP64, 1e-10), (Precision.INT8X4, 1e-8)):
+        rj = rimp2.spin_components(torch.from_numpy(w["rijk_b_ov"]).cuda(), w["rijk_eps"], 5, 19,
+                                   precision=prec)
+        assert abs(rj.e_total - float(w["rijk_e_corr_reference"])) <= tol
     # H2 / cc-pVDZ (rimp2.rs:3187-3201): within 1e-4 of canonical MP2, and the oracle
     nh, vh = int(w["h2_nocc"]), int(w["h2_nvir"])
     for prec in (Precision.FP64, Precision.INT8X4):

@@ -64,3 +64,14 @@ def test_h2_known_answer(water):
     # one occupied orbital: only the (0, 0) pair, whose same-spin part is zero
     e_os, e_ss, _ = oracle.spin_components(water["h2_b_ov"], water["h2_eps"], nocc, nvir)
     assert nocc == 1 and abs(e_ss) <= 1e-15 * abs(e_os)
+
+
+def test_rijk_water_known_answer(water):
+    """Water with a density-fitted SCF (def2-universal-jkfit) and cc-pVDZ-RI for
+    MP2: the reference's RI-JK known answer (testdata/reference/
+    h2o_cc-pvdz_rijk-def2-universal-jkfit_dfmp2.json, PySCF 2.13.1), which the
+    fixture's own integrals and SCF reproduce to 1e-12 Eh."""
+    assert abs(float(water["rijk_e_rhf"]) - float(water["rijk_e_rhf_reference"])) <= 1e-9
+    e = oracle.spin_components(water["rijk_b_ov"], water["rijk_eps"], 5, 19)[2]
+    assert abs(e - float(water["rijk_e_corr_reference"])) <= 1e-10
+    assert e == pytest.approx(float(water["rijk_e_corr_numpy"]), abs=1e-13)
