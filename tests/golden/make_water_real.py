@@ -24,6 +24,10 @@ Method (this file's own code):
     and (P|Q), the inputs of the dressing / MO-transform stage
     (stream_dressed_mo_band, rimp2.rs:648-756).
 
+The equilibrium water also in conventional (unfitted) MP2: B is then an exact
+factor of the four-centre (ia|jb) matrix, and the energy must meet the
+reference's testdata/reference/h2o_cc-pvdz_mp2.json (PySCF).
+
 Three molecules. H2 at 0.74 Angstrom, the reference's inline known-answer test
 (rimp2.rs:3187-3201: RI-MP2 within 1e-4 of the canonical MP2 -0.0263715576).
 The equilibrium water once more with a density-fitted SCF (J and K fitted in
@@ -64,6 +68,7 @@ REF_SCS = os.path.join(REF, "testdata", "reference", "validation", "scs_mp2_grad
                        "h2o_distorted_cc-pvdz.json")
 REF_RIJK = os.path.join(REF, "testdata", "reference",
                         "h2o_cc-pvdz_rijk-def2-universal-jkfit_dfmp2.json")
+REF_CONV = os.path.join(REF, "testdata", "reference", "h2o_cc-pvdz_mp2.json")
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "water_ccpvdz_real.npz")
 BOHR_ANGSTROM = 0.52917721092   # CODATA 2010, the reference's Å <-> Bohr constant
 Z = {"H": 1, "O": 8}
@@ -493,8 +498,20 @@ def build(xyz_path, text=None, jk_fit=None):
     co, cv = C[:, :nocc], C[:, nocc:]
     pia = np.einsum("Pmn,mi,na->Pia", j3, co, cv).reshape(naux, nocc * nvir)
     b = np.linalg.solve(np.linalg.cholesky(j2), pia)
+    # exact (ia|jb) of the SCF's own four-centre integrals, an (ov x ov) matrix
+    ovov = np.einsum("pqrs,pi,qa,rj,sb->iajb", eri, co, cv, co, cv,
+                     optimize=True).reshape(nocc * nvir, nocc * nvir)
     return dict(b_ov=b, eps=e_orb, nocc=nocc, nvir=nvir, naux=naux, e_rhf=e_rhf, enuc=enuc,
-                iterations=it + 1, j3=j3, c=C, j2=j2)
+                iterations=it + 1, j3=j3, c=C, j2=j2, ovov=ovov)
+
+
+def exact_factor(m):
+    """B (rank x n) with B^T B = m for a symmetric positive semidefinite m: the
+    eigenvectors scaled by the square roots of the eigenvalues, dropping those at
+    or below 1e-14 of the largest (round-off of a Gram matrix)."""
+    w, v = np.linalg.eigh(0.5 * (m + m.T))
+    keep = w > 1e-14 * w[-1]
+    return (v[:, keep] * np.sqrt(w[keep])).T.copy()
 
 
 def numpy_mp2(b, eps, nocc, nvir, fc=0):
@@ -578,6 +595,23 @@ def main():
     out.update(h2_b_ov=h["b_ov"], h2_eps=h["eps"], h2_nocc=h["nocc"], h2_nvir=h["nvir"],
                h2_e_rhf=h["e_rhf"], h2_e_corr_numpy=os_ + ss_, h2_e_os_numpy=os_,
                h2_e_corr_canonical=-0.0263715576)
+
+    # Conventional MP2 of the equilibrium water (testdata/reference/h2o_cc-pvdz_mp2.json,
+    # PySCF): B is an exact factor of the four-centre (ia|jb), so the same pair-energy
+    # path computes the unfitted energy.
+    cv_ = json.load(open(REF_CONV))
+    bx = exact_factor(w["ovov"])
+    err = np.max(np.abs(bx.T @ bx - w["ovov"]))
+    os_, ss_ = numpy_mp2(bx, w["eps"], w["nocc"], w["nvir"])
+    print(f"conventional water: rank {bx.shape[0]} of {bx.shape[1]}, |B^T B - (ia|jb)| {err:.1e}; "
+          f"E_corr {os_ + ss_:.12f} (PySCF {cv_['correlation_energy']:.12f}, diff "
+          f"{os_ + ss_ - cv_['correlation_energy']:.2e})", flush=True)
+    if abs(cv_["scf_energy"] - ref["rhf_energy"]) > 1e-9 or \
+            abs(os_ + ss_ - cv_["correlation_energy"]) > 1e-8:
+        sys.exit("conventional MP2 of the exact factor does not match the reference")
+    out.update(conv_b_ov=bx, conv_e_corr_numpy=os_ + ss_,
+               conv_e_corr_reference=cv_["correlation_energy"],
+               conv_e_rhf_reference=cv_["scf_energy"])
     np.savez_compressed(OUT, **out)
     print("wrote", OUT)
 

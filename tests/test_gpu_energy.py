@@ -189,8 +189,9 @@ def test_real_water_against_published_energies(oracle_mod):
     """cc-pVDZ / cc-pVDZ-RI water (tests/golden/water_ccpvdz_real.npz): the
     distorted geometry against ORCA/PySCF (1e-10 Eh, all-electron and one frozen
     core), the equilibrium geometry against the reference's known answer (1e-7),
-    H2 against the reference's H2 known answer (1e-4) in FP64 and INT8X4, and
-    the RI-JK water against its PySCF known answer (1e-10 Eh)."""
+    H2 against the reference's H2 known answer (1e-4) in FP64 and INT8X4, the
+    RI-JK water against its PySCF known answer (1e-10 Eh), and conventional MP2
+    of the equilibrium water from an exact factor of (ia|jb) (1e-9 Eh)."""
     import os
 
     w = np.load(os.path.join(os.path.dirname(__file__), "golden", "water_ccpvdz_real.npz"))
@@ -213,6 +214,11 @@ def test_real_water_against_published_energies(oracle_mod):
         rj = rimp2.spin_components(torch.from_numpy(w["rijk_b_ov"]).cuda(), w["rijk_eps"], 5, 19,
                                    precision=prec)
         assert abs(rj.e_total - float(w["rijk_e_corr_reference"])) <= tol
+    # conventional MP2 (PySCF): B^T B is the four-centre (ia|jb), 95 rows
+    for prec, tol in ((Precision.FP64, 1e-9), (Precision.INT8X4, 1e-8)):
+        cv = rimp2.spin_components(torch.from_numpy(w["conv_b_ov"]).cuda(), w["eps"], 5, 19,
+                                   precision=prec)
+        assert abs(cv.e_total - float(w["conv_e_corr_reference"])) <= tol
     # H2 / cc-pVDZ (rimp2.rs:3187-3201): within 1e-4 of canonical MP2, and the oracle
     nh, vh = int(w["h2_nocc"]), int(w["h2_nvir"])
     for prec in (Precision.FP64, Precision.INT8X4):
