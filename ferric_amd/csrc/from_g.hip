@@ -66,6 +66,15 @@ __device__ __forceinline__ void load_tiles(const FromGArgs &a, const double *blk
     }
 }
 
+// 1/d for a normal, nonzero d: the hardware reciprocal estimate refined by two
+// Newton-Raphson steps r <- r + r (1 - d r), each of which squares the relative
+// error, so the result is within an ulp or two of the rounded quotient.
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
 // One item = (pair, row tile A): the tile pairs (A, B) for B = A .. nt - 1 in
 // turn, the next pair of tiles loading into registers while the current one is
 // consumed from LDS. The loop reads nothing else from global memory, so the
@@ -99,8 +108,10 @@ __global__ __launch_bounds__(kThreads) void energy_from_g_kernel(FromGArgs a) {
         if (threadIdx.x < kT) eps_b[threadIdx.x] = veb;
         __syncthreads();
         if (tb + 1 < a.nt) load_tiles<kPer>(a, blk, A0, B0 + kT, va, vb, veb);
-        KSum &o = tb == ta ? osd : os;
-        KSum &q = tb == ta ? ssd : ss;
+        // The kPer terms of a thread are all of one sign (D < 0), so their plain
+        // sum is within kPer ulps of exact; it enters the compensated sums once
+        // per tile pair.
+        double to = 0.0, tq = 0.0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int e = threadIdx.x + k * kThreads;
@@ -108,11 +119,13 @@ __global__ __launch_bounds__(kThreads) void energy_from_g_kernel(FromGArgs a) {
             if (A0 + r >= a.nvir || B0 + c >= a.nvir) continue;
             const double g1 = cur[k];            // (ia|jb)
             const double g2 = tba[c * kLd + r];  // (ib|ja)
-            const double inv = 1.0 / (eij - eps_a[r] - eps_b[c]);
+            const double inv = recip(eij - eps_a[r] - eps_b[c]);
             const double dd = g1 - g2;
-            o.add((g1 * g1 + g2 * g2) * inv);
-            q.add((dd * dd) * inv);
+            to = fma(fma(g1, g1, g2 * g2), inv, to);
+            tq = fma(dd * dd, inv, tq);
         }
+        (tb == ta ? osd : os).add(to);
+        (tb == ta ? ssd : ss).add(tq);
         __syncthreads();
     }
     os.merge(0.5 * osd.s, 0.5 * osd.c);
