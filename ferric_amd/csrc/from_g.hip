@@ -79,6 +79,9 @@ __device__ __forceinline__ double recip(double d) {
 // turn, the next pair of tiles loading into registers while the current one is
 // consumed from LDS. The loop reads nothing else from global memory, so the
 // wait for the next tiles comes only at their LDS write.
+// kExact: 1/D by division, for denominator ranges that may hold zero or values
+// outside [2^-1000, 2^1000], where the refined reciprocal is not accurate.
+template <bool kExact>
 __global__ __launch_bounds__(kThreads) void energy_from_g_kernel(FromGArgs a) {
     // Only (B, A) goes through LDS (its transpose is read); each thread keeps
     // its own elements of (A, B) in registers, in the order it consumes them.
@@ -119,7 +122,8 @@ __global__ __launch_bounds__(kThreads) void energy_from_g_kernel(FromGArgs a) {
             if (A0 + r >= a.nvir || B0 + c >= a.nvir) continue;
             const double g1 = cur[k];            // (ia|jb)
             const double g2 = tba[c * kLd + r];  // (ib|ja)
-            const double inv = recip(eij - eps_a[r] - eps_b[c]);
+            const double d = eij - eps_a[r] - eps_b[c];
+            const double inv = kExact ? 1.0 / d : recip(d);
             const double dd = g1 - g2;
             to = fma(fma(g1, g1, g2 * g2), inv, to);
             tq = fma(dd * dd, inv, tq);
@@ -205,8 +209,19 @@ fh_status from_g_core(State &s, const double *g, bool dense, const int32_t *pair
     a.item_partials = (double *)s.slots[kSlotItems].ptr;
     mark(s, 0, stream);
     mark(s, 1, stream);
-    hipLaunchKernelGGL(energy_from_g_kernel, dim3((unsigned)nitems), dim3(kThreads), 0, stream,
-                       a);
+    // Every denominator lies in [2 min e_occ - 2 max e_vir, 2 max e_occ - 2 min e_vir]
+    // (over the active orbitals); the reciprocal kernel needs that range negative and
+    // of normal magnitude with margin.
+    const auto [omin, omax] = std::minmax_element(he.begin(), he.begin() + nocc);
+    const auto [vmin, vmax] = std::minmax_element(he.begin() + nocc, he.end());
+    const double dlo = 2.0 * *omin - 2.0 * *vmax, dhi = 2.0 * *omax - 2.0 * *vmin;
+    const bool exact = !(dhi < 0.0 && -dhi >= 0x1p-1000 && -dlo <= 0x1p1000);
+    if (exact)
+        hipLaunchKernelGGL(energy_from_g_kernel<true>, dim3((unsigned)nitems), dim3(kThreads), 0,
+                           stream, a);
+    else
+        hipLaunchKernelGGL(energy_from_g_kernel<false>, dim3((unsigned)nitems), dim3(kThreads), 0,
+                           stream, a);
     mark(s, 2, stream);
     double *dout = (double *)s.slots[kSlotPairs].ptr;
     hipLaunchKernelGGL(fold_pair_items_kernel, dim3((unsigned)npairs), dim3(256), 0, stream,

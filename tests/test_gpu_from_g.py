@@ -91,6 +91,33 @@ def test_non_gram_g_against_factor_brute_force(oracle_mod, nocc, nvir, fo):
     assert abs(wrong[1] - orc[1]) > 1e-9 * abs(orc[0])
 
 
+def test_dense_g_with_positive_and_tiny_denominators(oracle_mod):
+    """spin_components_from_g divides without checking the gap: with virtual
+    energies below some occupied ones (denominators of both signs) and with all
+    energies scaled to 1e-303 (denominators below 2^-1000) the kernel takes 1/D by
+    division and still equals the oracle's plain division to 1e-12."""
+    rng = np.random.default_rng(4242)
+    nocc, nvir, naux = 4, 70, 30
+    b = rng.standard_normal((naux, nocc * nvir)) * 0.05
+    c = rng.standard_normal((naux, nocc * nvir)) * 0.05
+    g = b.T @ c + 0.3 * (c.T @ b)
+    eps = np.concatenate([np.sort(rng.uniform(-1.0, 0.5, nocc)),
+                          np.sort(rng.uniform(-0.2, 2.0, nvir))])
+    for scale in (1.0, 1e-303):
+        e = eps * scale
+        eo, ev = e[:nocc], e[nocc:]
+        d = (eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None]
+             - ev[None, None, None, :])
+        assert d.min() < 0 < d.max() and np.all(d != 0)
+        print(f"from_g mixed-sign denominators, scale {scale}", flush=True)
+        gs = g * math.sqrt(scale)  # keeps g^2 / D of order one
+        sc = rimp2.spin_components_from_g(_cuda(gs), e, nocc, nvir)
+        orc = oracle_mod.spin_components_from_g(gs, e, nocc, nvir)
+        assert math.isfinite(sc.e_os) and math.isfinite(sc.e_ss)
+        assert abs(sc.e_os - orc[0]) <= 1e-12 * abs(orc[0])
+        assert abs(sc.e_ss - orc[1]) <= 1e-12 * abs(orc[0])
+
+
 def test_pair_blocks_from_domain_fits():
     a, v, xyz, cen, eps = synthetic.local_fit_chain(nocc=6, nvir=70, aux_per_site=5)
     nocc, nvir = 6, 70
