@@ -211,7 +211,9 @@ fh_status fh_rimp2_energy_dev(const void *b_dev, fh_dtype b_dtype, size_t naux, 
  *   deps[first_occ + k] = dE/deps_k = 2 P_kk (active occupied),
  *   deps[nocc_total + c] = 2 P_cc (virtual), 0 elsewhere (frozen core)
  *     (oo_rimp2.rs:1192-1300, rimp2.rs:2026-2052).
- * T~ is recomputed per (i,j) block and never stored for the whole problem.
+ * FP64 and BF16X3 recompute T~ per occupied row and never store it for the whole
+ * problem; INT8X4 holds the whole T~ in device memory as BF16 hi/lo pairs
+ * ((nocc nvir_pad)^2 x 4 bytes) for one GEMM Gamma = 4 B T~.
  * kappa must be NaN (None): FH_UNSUPPORTED otherwise.
  * Host-pointer version: b_ov, eps, gamma (naux*nov) and deps (n_eps) on the host. */
 fh_status fh_rimp2_energy_adjoint(const double *b_ov, size_t naux, size_t nocc, size_t nvir,
