@@ -170,9 +170,13 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
         rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
         cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
     }
-    bool full = true;
+    bool full = true, anyr = false, anyc = false;
 #pragma unroll
-    for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
+    for (int k = 0; k < NB; ++k) {
+        full = full && rv[k] && cv[k];
+        anyr = anyr || rv[k];
+        anyc = anyc || cv[k];
+    }
     v4i acc[kLevels][NB][NB];
 #pragma unroll
     for (int l = 0; l < kLevels; ++l)
@@ -182,14 +186,19 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
             for (int y = 0; y < NB; ++y) acc[l][x][y] = (v4i){0, 0, 0, 0};
 
     v4i fa[2][4][NB], fb[2][4][NB];
-    auto fetch = [&](int P, v4i(&ra)[4][NB], v4i(&rb)[4][NB]) {
+    // FULL: every block is loaded. Otherwise only the blocks inside nvir_pad,
+    // the only ones the MFMAs below read (a 16-row block is used when both its
+    // row block and its column block are inside).
+    auto fetch = [&](int P, v4i(&ra)[4][NB], v4i(&rb)[4][NB], auto fullc) {
         const int64_t off = (int64_t)P * FB;
 #pragma unroll
         for (int d = 0; d < 4; ++d)
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                ra[d][k] = *(const v4i *)(pa[k] + off + d * 1024);
-                rb[d][k] = *(const v4i *)(pb[k] + off + d * 1024);
+                if (decltype(fullc)::value || rv[k])
+                    ra[d][k] = *(const v4i *)(pa[k] + off + d * 1024);
+                if (decltype(fullc)::value || cv[k])
+                    rb[d][k] = *(const v4i *)(pb[k] + off + d * 1024);
             }
     };
     // FULL: every 16x16 block of the wave lies inside nvir_pad (the common
@@ -209,24 +218,27 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
                                     ra[p][x], rb[q][y], acc[p + q][x][y], 0, 0, 0);
                 }
     };
-    auto compute = [&](const v4i(&ra)[4][NB], const v4i(&rb)[4][NB]) {
-        if (full)
-            mma(ra, rb, std::true_type{});
-        else
-            mma(ra, rb, std::false_type{});
-    };
     // Iteration p covers panel p * pstep + poff (a single-product item's odd
     // panel past the end is skipped by its waves); unrolled by two so the
     // register buffer of each step is static.
     const int iters = sq.iters;
     auto valid = [&](int p) { return p < iters && p * pstep + poff < np; };
-    if (valid(0)) fetch(poff, fa[0], fb[0]);
-    for (int p = 0; p < iters; p += 2) {
-        if (valid(p + 1)) fetch((p + 1) * pstep + poff, fa[1], fb[1]);
-        if (valid(p)) compute(fa[0], fb[0]);
-        if (valid(p + 2)) fetch((p + 2) * pstep + poff, fa[0], fb[0]);
-        if (valid(p + 1)) compute(fa[1], fb[1]);
-    }
+    auto run = [&](auto fullc) {
+        if (valid(0)) fetch(poff, fa[0], fb[0], fullc);
+        for (int p = 0; p < iters; p += 2) {
+            if (valid(p + 1)) fetch((p + 1) * pstep + poff, fa[1], fb[1], fullc);
+            if (valid(p)) mma(fa[0], fb[0], fullc);
+            if (valid(p + 2)) fetch((p + 2) * pstep + poff, fa[0], fb[0], fullc);
+            if (valid(p + 1)) mma(fa[1], fb[1], fullc);
+        }
+    };
+    // A wave of a last tile whose quadrant lies wholly past nvir_pad (no row
+    // block or no column block inside) has nothing to contract: its level sums
+    // stay zero.
+    if (full)
+        run(std::true_type{});
+    else if (anyr && anyc)
+        run(std::false_type{});
     __syncthreads();  // eps_t / rexp_t written; every wave done with its loop
 
     constexpr int LDG = TILE + 1;
