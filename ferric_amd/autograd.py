@@ -41,7 +41,9 @@ class RiMp2Function(torch.autograd.Function):
         eps = torch.cat([eps_occ.detach(), eps_vir.detach()]).to("cpu", torch.float64).numpy()
         _, _, gamma, deps = energy_adjoint(b, eps, nocc, nvir, 0, nocc, float(g[0]), float(g[1]),
                                            _adjoint_precision(ctx.precision))
-        gb = gamma.to(b.dtype) if ctx.needs_input_grad[0] else None
+        if not torch.is_tensor(gamma):  # host entry (CPU b_ov): a numpy array
+            gamma = torch.from_numpy(gamma)
+        gb = gamma.to(b.device, b.dtype) if ctx.needs_input_grad[0] else None
         d = torch.from_numpy(deps)
         go = d[:nocc].to(eps_occ.device, eps_occ.dtype) if ctx.needs_input_grad[1] else None
         gv = d[nocc:].to(eps_vir.device, eps_vir.dtype) if ctx.needs_input_grad[2] else None

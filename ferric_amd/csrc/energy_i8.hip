@@ -88,7 +88,7 @@ __device__ __forceinline__ void decode_tile_pair_banded(int t, int nt, int &ta, 
 
 // sum_L 2^(8 (8 - L)) S_L * 2^(e - 60), summed from the smallest level up.
 __device__ __forceinline__ double combine_levels(int s2, int s3, int s4, int s5, int s6, int e) {
-    if (e >= kI8NonFinite) return NAN;
+    if (e >= kI8NonFiniteSum) return NAN;  // either row non-finite
     double t = (double)s6 * 0x1p-32;
     t += (double)s5 * 0x1p-24;
     t += (double)s4 * 0x1p-16;

@@ -57,8 +57,12 @@ __host__ __device__ inline int fp8_scale_exponent(float amax) {
 // ---- INT8 digit slices -----------------------------------------------------
 
 // Row exponent of a packed row holding a non-finite value (its digits are 0
-// and its energy terms come out NaN).
-constexpr int kI8NonFinite = 1 << 20;
+// and its energy terms come out NaN). The epilogues see the SUM of two row
+// exponents; finite rows have |e| < 1100, so a sum holding the marker is at
+// least 2^28 - 1100 and one without it stays below 2200: any sum at or above
+// kI8NonFiniteSum carries a non-finite row, whichever operand it came from.
+constexpr int kI8NonFinite = 1 << 28;
+constexpr int kI8NonFiniteSum = 1 << 27;
 // Largest naux_pad whose level sums fit int32: with |d1| <= 127 and the
 // other digits in [-128, 127], one level sums at most
 // 2 * 127 * 128 + 2 * 128 * 128 = 65280 per Q, and 65280 * 32832 < 2^31.

@@ -66,6 +66,10 @@ def lib():
                                         ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         L.orc_energy_bound.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_energy_bound_row.argtypes = [_dp, _dp, _sz, _sz, _sz, _dp, _sz, _sz,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, _dp]
+        L.orc_energy_bound_finish.argtypes = [_dp, _sz, _sz, _dp]
         L.orc_u_same_spin.argtypes = [_dp, _sz, _sz, _sz, _dp, _sz, _sz, _sz, ctypes.c_int, _dp]
         L.orc_u_opposite_spin.argtypes = [_dp, _dp, _sz, _sz, _sz, _sz, _sz, _dp, _sz, _sz, _sz,
                                           _dp, _sz, _sz, _sz, ctypes.c_int, _dp]
@@ -248,6 +252,45 @@ def energy_bound(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, kappa=None
     lib().orc_energy_bound(_p(b), naux, nocc, nvir, _p(e), first_occ, nocc_total, hk, k,
                            float(eps_g), float(eta), _p(out))
     return float(out[0]), float(out[1])
+
+
+def row_partials_blas(b_ov, eps, nocc, nvir, rows, first_occ=0, nocc_total=None, kappa=None,
+                      eps_g=None, eta=0.0, bounds=True):
+    """Per occupied row i in ``rows``: the oracle's partial (e_os_i, e_ss_i) of
+    spin_components_from_blocks (rimp2.rs:1520-1530) from a BLAS tail block
+    G_i = B_i^T B_tail, and the energy bound of that row alone (the bound of
+    ``energy_bound`` restricted to the row's pairs, with |B|^T|B| as S_ab).
+
+    Returns {i: (e_os_i, e_ss_i, bound_os_i, bound_ss_i)}. Rows are computed one
+    at a time, so the memory is two (nvir, (nocc - i) nvir) blocks."""
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    naux = b.shape[0]
+    if eps_g is None:
+        u = 2.0 ** -53
+        eps_g = 2.0 * naux * u / (1.0 - naux * u)
+    hk, k = _kappa_args(kappa)
+    out = {}
+    for i in rows:
+        bi = b[:, i * nvir:(i + 1) * nvir]
+        tail = b[:, i * nvir:]
+        g = np.ascontiguousarray(bi.T @ tail)
+        eos, ess = i_partial(g, i, nocc, nvir, e, first_occ, nocc_total, kappa)
+        bos = bss = float("nan")
+        if bounds:
+            s = np.ascontiguousarray(np.abs(bi).T @ np.abs(tail))
+            acc = np.zeros(4)
+            lib().orc_energy_bound_row(_p(g), _p(s), i, nocc, nvir, _p(e), first_occ, nocc_total,
+                                       hk, k, float(eps_g), float(eta), _p(acc))
+            bnd = np.zeros(2)
+            lib().orc_energy_bound_finish(_p(acc), nocc, nvir, _p(bnd))
+            bos, bss = float(bnd[0]), float(bnd[1])
+            del s
+        out[int(i)] = (eos, ess, bos, bss)
+        del g
+    return out
 
 
 def u_same_spin(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, nthreads: int = 0):

@@ -424,6 +424,54 @@ int orc_rimp2_adjoint(const double *b, size_t naux, size_t nocc, size_t nvir, co
  * plus the f64 summation term gamma_m (sum|t| + sum|t^|), m = nvir^2 + 2 nocc + 8.
  * bound = {os, ss}.
  * ------------------------------------------------------------------------- */
+/* The bound's sums for one occupied row i from its tail blocks g_i
+ * (= B_i^T B_tail) and s_i (= |B_i|^T |B_tail|), both laid out as
+ * orc_tail_block writes them: acc = {bos, bss, abs_os, abs_ss} of that row. */
+void orc_energy_bound_row(const double *g, const double *s, size_t i, size_t nocc, size_t nvir,
+                          const double *eps, size_t first_occ, size_t nocc_total, int has_kappa,
+                          double kappa, double eps_g, double eta, double *acc) {
+    const size_t ld = (nocc - i) * nvir;
+    double bos = 0.0, bss = 0.0, abs_os = 0.0, abs_ss = 0.0;
+    for (size_t j = i; j < nocc; ++j) {
+        const double fac = (i == j) ? 1.0 : 2.0;
+        const size_t jcol = (j - i) * nvir;
+        const double e_ij = eps[first_occ + i] + eps[first_occ + j];
+        for (size_t a = 0; a < nvir; ++a)
+            for (size_t bb = 0; bb < nvir; ++bb) {
+                const double gab = fabs(g[a * ld + jcol + bb]);
+                const double gba = fabs(g[bb * ld + jcol + a]);
+                const double dab = eps_g * s[a * ld + jcol + bb] + eta;
+                const double dba = eps_g * s[bb * ld + jcol + a] + eta;
+                const double denom = e_ij - eps[nocc_total + a] - eps[nocc_total + bb];
+                double damp = 1.0;
+                if (has_kappa) {
+                    const double d1 = 1.0 - exp(kappa * denom);
+                    damp = d1 * d1;
+                }
+                const double ad = fabs(denom);
+                bos += fac * damp * dab * (2.0 * gab + dab) / ad;
+                bss += fac * damp * (dab * (2.0 * gab + dab) + dab * gba + gab * dba +
+                                     dab * dba) / ad;
+                abs_os += fac * damp * gab * gab / ad;
+                abs_ss += fac * damp * gab * fabs(gab - gba) / ad;
+            }
+    }
+    acc[0] = bos;
+    acc[1] = bss;
+    acc[2] = abs_os;
+    acc[3] = abs_ss;
+}
+
+/* The f64 summation term gamma_m with m = nvir^2 + 2 nocc + 8 applied to the
+ * sums acc = {bos, bss, abs_os, abs_ss}: bound = {os, ss}. */
+void orc_energy_bound_finish(const double *acc, size_t nocc, size_t nvir, double *bound) {
+    const double u = ldexp(1.0, -53);
+    const double m = (double)(nvir * nvir + 2 * nocc + 8);
+    const double gam = m * u / (1.0 - m * u);
+    bound[0] = acc[0] + gam * (2.0 * acc[2] + acc[0]);
+    bound[1] = acc[1] + gam * (2.0 * acc[3] + acc[1]);
+}
+
 int orc_energy_bound(const double *b, size_t naux, size_t nocc, size_t nvir, const double *eps,
                      size_t first_occ, size_t nocc_total, int has_kappa, double kappa,
                      double eps_g, double eta, double *bound) {
@@ -431,7 +479,7 @@ int orc_energy_bound(const double *b, size_t naux, size_t nocc, size_t nvir, con
     double *absb = (double *)malloc(sizeof(double) * naux * nov);
     if (!absb) return ORC_ALLOC;
     for (size_t q = 0; q < naux * nov; ++q) absb[q] = fabs(b[q]);
-    double bos = 0.0, bss = 0.0, abs_os = 0.0, abs_ss = 0.0;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
     for (size_t i = 0; i < nocc; ++i) {
         const size_t ld = (nocc - i) * nvir;
         double *g = (double *)malloc(sizeof(double) * nvir * ld);
@@ -444,39 +492,15 @@ int orc_energy_bound(const double *b, size_t naux, size_t nocc, size_t nvir, con
         }
         orc_tail_block(b, naux, nocc, nvir, i, g);
         orc_tail_block(absb, naux, nocc, nvir, i, s);
-        for (size_t j = i; j < nocc; ++j) {
-            const double fac = (i == j) ? 1.0 : 2.0;
-            const size_t jcol = (j - i) * nvir;
-            const double e_ij = eps[first_occ + i] + eps[first_occ + j];
-            for (size_t a = 0; a < nvir; ++a)
-                for (size_t bb = 0; bb < nvir; ++bb) {
-                    const double gab = fabs(g[a * ld + jcol + bb]);
-                    const double gba = fabs(g[bb * ld + jcol + a]);
-                    const double dab = eps_g * s[a * ld + jcol + bb] + eta;
-                    const double dba = eps_g * s[bb * ld + jcol + a] + eta;
-                    const double denom = e_ij - eps[nocc_total + a] - eps[nocc_total + bb];
-                    double damp = 1.0;
-                    if (has_kappa) {
-                        const double d1 = 1.0 - exp(kappa * denom);
-                        damp = d1 * d1;
-                    }
-                    const double ad = fabs(denom);
-                    bos += fac * damp * dab * (2.0 * gab + dab) / ad;
-                    bss += fac * damp * (dab * (2.0 * gab + dab) + dab * gba + gab * dba +
-                                         dab * dba) / ad;
-                    abs_os += fac * damp * gab * gab / ad;
-                    abs_ss += fac * damp * gab * fabs(gab - gba) / ad;
-                }
-        }
+        double acc[4];
+        orc_energy_bound_row(g, s, i, nocc, nvir, eps, first_occ, nocc_total, has_kappa, kappa,
+                             eps_g, eta, acc);
+        for (int k = 0; k < 4; ++k) tot[k] += acc[k];
         free(g);
         free(s);
     }
     free(absb);
-    const double u = ldexp(1.0, -53);
-    const double m = (double)(nvir * nvir + 2 * nocc + 8);
-    const double gam = m * u / (1.0 - m * u);
-    bound[0] = bos + gam * (2.0 * abs_os + bos);
-    bound[1] = bss + gam * (2.0 * abs_ss + bss);
+    orc_energy_bound_finish(tot, nocc, nvir, bound);
     return ORC_OK;
 }
 

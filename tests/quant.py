@@ -114,10 +114,16 @@ def int8x4_g_rect(ba, bb):
     ea, da = int8x4_digits(ba)
     eb, db = int8x4_digits(bb)
     s = [np.zeros((da.shape[2], db.shape[2]), dtype=np.int64) for _ in range(5)]
+    # Each digit product sum is an integer of magnitude <= 128^2 naux < 2^53,
+    # and so is every partial sum of it: an FP64 BLAS product forms it exactly
+    # in any association (and much faster than an int64 matmul).
+    fa = da.astype(np.float64)
+    fb = db.astype(np.float64)
+    assert 128 ** 2 * max(da.shape[1], 1) < 2 ** 53
     for p in range(4):
         for q in range(4):
             if p + q < 5:
-                s[p + q] += da[p].T @ db[q]
+                s[p + q] += (fa[p].T @ fb[q]).astype(np.int64)
     t = s[4].astype(np.float64) * 2.0 ** -32
     t = t + s[3].astype(np.float64) * 2.0 ** -24
     t = t + s[2].astype(np.float64) * 2.0 ** -16

@@ -105,3 +105,34 @@ def test_range_and_non_finite():
     print("int8x4 nan", flush=True)
     sc = _gpu(b, eps, nocc, nvir)
     assert np.isnan(sc.e_total)
+
+
+def test_non_finite_row_meets_finite_rows():
+    """A NaN in one packed row (i, a) reaches every energy that reads it, also
+    where it meets a finite row of another occupied (pair (i, j), i != j) or of
+    the other spin channel: the row exponent marking it must survive the sum of
+    two exponents the epilogue forms, whatever the finite row's exponent."""
+    from ferric_amd import dlpno
+    from ferric_amd import unrestricted as u
+
+    rng = np.random.default_rng(11)
+    nocc, nvir, naux = 3, 20, 96
+    eps = np.concatenate([[-1.2, -1.0, -0.8], np.linspace(0.3, 1.5, nvir)])
+    b = rng.standard_normal((naux, nocc * nvir)) * 0.03   # row maxima ~0.1: exponents ~ -3
+    b[5, 1 * nvir + 7] = np.nan                           # occupied 1, virtual 7
+    print("int8x4 nan pairs", flush=True)
+    pe = dlpno.estimate_pair_energies_from_b_ov(b, eps, nocc, nvir, precision=I8)
+    for i in range(nocc):
+        for j in range(i, nocc):
+            touches = 1 in (i, j)
+            assert np.isnan(pe.e_os[i, j]) == touches, (i, j, pe.e_os)
+            assert np.isnan(pe.e_ss[i, j]) == touches, (i, j, pe.e_ss)
+    print("int8x4 nan opposite spin", flush=True)
+    beta = rng.standard_normal((naux, 2 * 24)) * 0.03
+    eps_b = np.concatenate([[-1.1, -0.9], np.linspace(0.25, 1.4, 24)])
+    ca = u.SpinChannel(b, eps, nocc, nvir)
+    cb = u.SpinChannel(beta, eps_b, 2, 24)
+    assert np.isnan(u.u_opposite_spin(ca, cb, precision=I8))
+    clean = b.copy()
+    clean[5, 1 * nvir + 7] = 0.0
+    assert np.isfinite(u.u_opposite_spin(u.SpinChannel(clean, eps, nocc, nvir), cb, precision=I8))

@@ -222,6 +222,79 @@ def test_energy_bound_is_positive_and_linear(oracle_mod, golden):
     assert b2[0] == pytest.approx(2 * b1[0], rel=1e-3)
 
 
+def _sign_aligned_energy_shift(oracle_mod, c, eps_g):
+    """E(G + delta) - E(G) for the per-element perturbation of the bound's
+    model with every element pushed away from zero: delta = sign(g) eps_g S."""
+    b, eps = c["b"], c["eps"]
+    nocc, nvir, fc, nt = c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"]
+    g = b.T @ b
+    s = np.abs(b).T @ np.abs(b)
+    e0 = oracle_mod.spin_components_from_g(g, eps, nocc, nvir, fc, nt)
+    e1 = oracle_mod.spin_components_from_g(g + np.sign(g) * eps_g * s, eps, nocc, nvir, fc, nt)
+    return e1[0] - e0[0], e1[1] - e0[1]
+
+
+@pytest.mark.parametrize("name", ["res_120_7_23_1_3", "res_150_9_31_2_5", "water_shapes"])
+def test_energy_bound_is_attained_not_loose(oracle_mod, golden, name):
+    """The bound of tests/common/rimp2_error_bound.rs (restated) is a worst case
+    over per-element G perturbations |delta_ab| <= eps_g S_ab. Pushing every
+    element away from zero by exactly eps_g S_ab raises every g_ab^2, so the
+    opposite-spin shift equals the opposite-spin bound up to its summation term
+    (a bound loose by a factor, a dropped f_ij = 2 or a wrong S fails); the
+    same-spin shift stays inside its bound and reaches a sizeable part of it."""
+    c = golden[name]
+    args = (c["b"], c["eps"], c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"])
+    eps_g = 1e-6
+    bos, bss = oracle_mod.energy_bound(*args, eps_g=eps_g)
+    dos, dss = _sign_aligned_energy_shift(oracle_mod, c, eps_g)
+    assert dos < 0
+    assert 0.999 * bos <= abs(dos) <= bos, (dos, bos)
+    assert 0.05 * bss <= abs(dss) <= bss, (dss, bss)
+
+
+def test_energy_bound_rows_sum_to_the_bound(oracle_mod, golden):
+    """The per-row bound and partials (row_partials_blas, used at sizes where
+    the whole bound is too slow) add up to the whole-problem bound and energy."""
+    c = golden["res_150_9_31_2_5"]
+    args = (c["b"], c["eps"], c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"])
+    whole = oracle_mod.energy_bound(*args)
+    e = oracle_mod.spin_components(*args)
+    rows = oracle_mod.row_partials_blas(c["b"], c["eps"], c["nocc"], c["nvir"],
+                                        range(c["nocc"]), c["first_occ"], c["nocc_total"])
+    assert sum(r[2] for r in rows.values()) == pytest.approx(whole[0], rel=1e-12)
+    assert sum(r[3] for r in rows.values()) == pytest.approx(whole[1], rel=1e-12)
+    assert sum(r[0] for r in rows.values()) == pytest.approx(e[0], rel=1e-12)
+    assert sum(r[1] for r in rows.values()) == pytest.approx(e[1], rel=1e-12)
+
+
+def test_fp32_accumulation_model_bounds_a_sequential_fp32_dot():
+    """The element model the reduced-precision tests use, eps_g = (naux + 16)
+    2^-24 (Higham's gamma_n for an FP32 dot product of length naux, with slack
+    for the rounding of the FP32 sum into FP64), bounds an FP32 dot product
+    accumulated in the worst association (one long sequential chain), including
+    on the hot-row (x20) distribution; the realised error is within three
+    orders of the bound (random rounding grows like sqrt(naux))."""
+    rng = np.random.default_rng(5)
+    naux = 480
+    worst_ratio = 0.0
+    for _ in range(200):
+        x = rng.standard_normal(naux).astype(np.float32)
+        y = rng.standard_normal(naux).astype(np.float32)
+        hot = rng.choice(naux, 5, replace=False)
+        x[hot] *= np.float32(20.0)
+        y[hot] *= np.float32(20.0)
+        acc = np.float32(0.0)
+        for q in range(naux):
+            acc = np.float32(acc + np.float32(x[q] * y[q]))
+        exact = math.fsum(float(a) * float(b) for a, b in zip(x, y))
+        s = float(np.abs(x.astype(np.float64)) @ np.abs(y.astype(np.float64)))
+        bound = (naux + 16) * 2.0 ** -24 * s
+        err = abs(float(acc) - exact)
+        assert err <= bound
+        worst_ratio = max(worst_ratio, err / bound)
+    assert worst_ratio >= 1e-3, worst_ratio
+
+
 # -- adjoint --------------------------------------------------------------------
 
 def test_x_ov_matches_scalar_definition(oracle_mod):
