@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "decane", "water"])
-    ap.add_argument("--variants", default="FP64,FP32,BF16X3,FP8_E4M3,INT8X4")
+    ap.add_argument("--variants", default="FP64,FP32,MIXED,BF16X3,FP8_E4M3,INT8X4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")

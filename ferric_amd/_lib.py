@@ -38,6 +38,7 @@ class Precision(enum.IntEnum):
     BF16X3 = 2
     FP8_E4M3 = 3
     INT8X4 = 4
+    MIXED = 5    # FP32 over 64-row Q panels, FP64 across panels (the reference's Mixed)
 
 
 class DType(enum.IntEnum):
@@ -55,7 +56,7 @@ class StatsC(ctypes.Structure):
         "calls", "offloaded", "fp64_calls", "fp32_calls", "bf16x3_calls", "fp8_calls",
         "adjoint_calls", "batched_calls", "h2d_bytes", "d2h_bytes", "kernel_launches",
         "fail_layout", "fail_pool_full", "fail_hip", "fail_range", "fail_unsupported",
-        "pool_peak_bytes", "int8x4_calls")]
+        "pool_peak_bytes", "int8x4_calls", "mixed_calls")]
 
 
 class TimingC(ctypes.Structure):
@@ -190,7 +191,7 @@ SIGNATURES = {
 }
 
 _lib = None
-ABI_VERSION = 9   # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 10  # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

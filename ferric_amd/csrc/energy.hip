@@ -55,15 +55,15 @@ __device__ __forceinline__ pair::Item decode_item(const EnergyArgs &a) {
     return it;
 }
 
-template <typename T, int TILE, int KP, bool ADJ, bool SIGNED = false>
+template <typename T, int TILE, int KP, bool ADJ, bool SIGNED = false, bool FLUSH = false>
 __global__ __launch_bounds__(256) void pair_k4_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::K4Lds<T, TILE, KP> lds;
-    pair::pair_k4_body<T, TILE, KP, ADJ, false, SIGNED>(a, decode_item(a), lds);
+    pair::pair_k4_body<T, TILE, KP, ADJ, false, SIGNED, FLUSH>(a, decode_item(a), lds);
 }
 
 // Opposite-spin rectangle: item (i, J, tA, tB), J and tB on the beta side;
 // the items of alpha row i are contiguous.
-template <typename T, int TILE, int KP>
+template <typename T, int TILE, int KP, bool FLUSH = false>
 __global__ __launch_bounds__(256) void pair_rect_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::K4Lds<T, TILE, KP> lds;
     pair::Item it;
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void pair_rect_kernel(EnergyArgs a) {
     it.j = (int)(p % a.nocc_b);
     it.ta = t / a.ntile_b;
     it.tb = t % a.ntile_b;
-    pair::pair_k4_body<T, TILE, KP, false, true>(a, it, lds);
+    pair::pair_k4_body<T, TILE, KP, false, true, false, FLUSH>(a, it, lds);
 }
 
 template <int TILE, bool ADJ>
@@ -164,6 +164,11 @@ hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
         else if (a.tile == 64) e = launch_items(pair_k4_kernel<float, 64, 16, false>, a, st);
         else if (a.tile == 32) e = launch_items(pair_k4_kernel<float, 32, 16, false>, a, st);
         break;
+    case Prec::MIXED:
+        if (a.tile == 128) e = launch_items(pair_k4_kernel<float, 128, 16, false, false, true>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_k4_kernel<float, 64, 16, false, false, true>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_k4_kernel<float, 32, 16, false, false, true>, a, st);
+        break;
     case Prec::BF16X3:
         if (a.tile == 128) e = launch_items(pair_bf16x3_kernel<128, false>, a, st);
         else if (a.tile == 64) e = launch_items(pair_bf16x3_kernel<64, false>, a, st);
@@ -218,6 +223,11 @@ hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st) {
         if (a.tile == 128) e = launch_items(pair_rect_kernel<float, 128, 16>, a, st);
         else if (a.tile == 64) e = launch_items(pair_rect_kernel<float, 64, 16>, a, st);
         else if (a.tile == 32) e = launch_items(pair_rect_kernel<float, 32, 16>, a, st);
+        break;
+    case Prec::MIXED:
+        if (a.tile == 128) e = launch_items(pair_rect_kernel<float, 128, 16, true>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_rect_kernel<float, 64, 16, true>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_rect_kernel<float, 32, 16, true>, a, st);
         break;
     case Prec::I8X4:
         e = launch_energy_i8x4_rect(a, st);

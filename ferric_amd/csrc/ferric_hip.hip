@@ -230,6 +230,7 @@ size_t packed_elem_bytes(Prec p) {
     switch (p) {
     case Prec::F64: return 8;
     case Prec::F32: return 4;
+    case Prec::MIXED: return 4;
     case Prec::BF16X3: return 2;  // per array (hi and lo)
     case Prec::FP8: return 1;
     case Prec::I8X4: return 4;  // four digit bytes
@@ -245,9 +246,10 @@ size_t packed_op_bytes(const EnergyPlan &pl, size_t nocc, Prec prec) {
 }
 
 // Reduced-precision range check (the reference's check_f32_arithmetic_range,
-// rimp2_gpu.rs:280-293, with the whole naux-deep FP32 accumulation as the
-// panel): refuse when max|B| > sqrt(FLT_MAX / naux).
-fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t naux,
+// rimp2_gpu.rs:280-293): refuse when max|B| > sqrt(FLT_MAX / depth), depth
+// the Q rows of one FP32 accumulation (the whole naux for the FP32-accumulating
+// modes, the 64-row panel for MIXED, as the reference's b).
+fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t depth,
                       hipStream_t stream) {
     unsigned int *dbits = (unsigned int *)s.slots[kSlotScalar].ptr;
     fh_status st = hip_check(launch_absmax(b, in_f32, (int64_t)n, dbits, stream), "absmax");
@@ -259,12 +261,12 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t nau
     if (st != FH_OK) return st;
     float m;
     memcpy(&m, &bits, 4);
-    const double limit = std::sqrt((double)std::numeric_limits<float>::max() / (double)naux);
+    const double limit = std::sqrt((double)std::numeric_limits<float>::max() / (double)depth);
     if ((double)m > limit)
         return fail(FH_RANGE,
-                    "max|B_ov| = %e exceeds sqrt(FLT_MAX/naux) = %e: an FP32 accumulation could "
-                    "overflow",
-                    (double)m, limit);
+                    "max|B_ov| = %e exceeds sqrt(FLT_MAX/%zu) = %e: an FP32 accumulation of "
+                    "%zu products could overflow",
+                    (double)m, depth, limit, depth);
     return FH_OK;
 }
 
@@ -320,7 +322,8 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     // INT8 slices carry their own power-of-two scale per row: any finite
     // value is in range.
     if (prec != Prec::F64 && prec != Prec::I8X4) {
-        const size_t depth = naux + (signed_k ? naux_neg : 0);
+        const size_t depth = prec == Prec::MIXED ? std::min(naux, (size_t)kMixedPanel)
+                                                 : naux + (signed_k ? naux_neg : 0);
         st = range_check(s, b_dev, in_f32, naux * nov, depth, stream);
         if (st != FH_OK) return st;
         if (signed_k) {
@@ -347,7 +350,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     pa.nvir_pad = pl.nvir_pad;
     pa.naux_pad = pl.naux_pad;
     pa.fmt = prec == Prec::F64      ? PackFmt::F64
-             : prec == Prec::F32    ? PackFmt::F32
+             : prec == Prec::F32 || prec == Prec::MIXED ? PackFmt::F32
              : prec == Prec::BF16X3 ? PackFmt::BF16X2
              : prec == Prec::I8X4   ? PackFmt::I8X4
                                     : PackFmt::FP8;
@@ -455,6 +458,7 @@ void commit(State &s, Prec prec, bool adjoint, bool batched) {
     switch (prec) {
     case Prec::F64: s.stats.fp64_calls++; break;
     case Prec::F32: s.stats.fp32_calls++; break;
+    case Prec::MIXED: s.stats.mixed_calls++; break;
     case Prec::BF16X3: s.stats.bf16x3_calls++; break;
     case Prec::FP8: s.stats.fp8_calls++; break;
     case Prec::I8X4: s.stats.int8x4_calls++; break;
@@ -502,6 +506,7 @@ bool to_prec(int p, Prec *out) {
     case FH_BF16X3: *out = Prec::BF16X3; return true;
     case FH_FP8_E4M3: *out = Prec::FP8; return true;
     case FH_INT8X4: *out = Prec::I8X4; return true;
+    case FH_MIXED: *out = Prec::MIXED; return true;
     default: return false;
     }
 }

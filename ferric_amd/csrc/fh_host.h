@@ -102,7 +102,7 @@ fh_status hip_check(hipError_t e, const char *what);
 fh_status check_shapes(size_t naux, size_t nocc, size_t nvir, size_t n_eps, size_t first_occ,
                        size_t nocc_total, bool *empty);
 fh_status check_kappa(double kappa, bool *has);
-fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t naux,
+fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t depth,
                       hipStream_t stream);
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec);
 int tuned_tile(Prec prec, size_t nvir, size_t naux);

@@ -94,13 +94,21 @@ hipError_t launch_absmax(const void *b, bool in_f32, int64_t n, unsigned int *ou
 
 // ---- Energy ----------------------------------------------------------------
 
-enum class Prec { F64 = 0, F32 = 1, BF16X3 = 2, FP8 = 3, I8X4 = 4 };
+// MIXED: FP32 operands and FP32 MFMA accumulation over Q panels of at most
+// kMixedPanel rows, each panel's sums added into FP64 accumulators (the
+// reference's Precision::Mixed, crates/ferric-core/src/gpu/mixed.rs:70,
+// 122-180).
+enum class Prec { F64 = 0, F32 = 1, BF16X3 = 2, FP8 = 3, I8X4 = 4, MIXED = 5 };
+
+// Q rows of one FP32 partial sum in MIXED (MIXED_K_PANEL_DEFAULT).
+constexpr int kMixedPanel = 64;
 
 // Panel depth along Q of each precision's kernel (naux_pad is a multiple).
 inline int panel_depth(Prec p) {
     switch (p) {
     case Prec::F64: return 16;
     case Prec::F32: return 16;
+    case Prec::MIXED: return 16;
     case Prec::BF16X3: return 32;
     case Prec::FP8: return 128;
     case Prec::I8X4: return 64;

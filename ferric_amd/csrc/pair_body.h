@@ -401,9 +401,18 @@ using K4Lds = T[2][4][TILE][KP + 16 / sizeof(T)];
 // belong to the negative factor of a signed kernel; their A-side fragments are
 // negated, so the MFMA chain accumulates B+^T B+ - B-^T B- (a compile-time
 // switch, so the other kernels carry no extra registers for it).
-template <typename T, int TILE, int KP, bool ADJ, bool RECT = false, bool SIGNED = false>
+// FLUSH (FP32 only, the MIXED mode): the FP32 accumulators restart every
+// kMixedPanel Q rows and each panel's sums are added into FP64 accumulators,
+// which the epilogue reads; a two-product item's accumulators each cover a
+// whole 64-row panel, a single-product item's (even / odd 16-row panels) 32
+// rows of it.
+template <typename T, int TILE, int KP, bool ADJ, bool RECT = false, bool SIGNED = false,
+          bool FLUSH = false>
 __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it,
                                              K4Lds<T, TILE, KP> &lds) {
+    static_assert(!FLUSH || (std::is_same<T, float>::value && !ADJ && !SIGNED),
+                  "the FP64 flush is an FP32 energy mode");
+    static_assert(kMixedPanel % (2 * KP) == 0, "flushes fall on whole loop iterations");
     using M = Mfma4<T>;
     constexpr int TW = TILE / 2;          // wave sub-tile side (2x2 waves)
     constexpr int NB = TW / 16;           // 16x16 blocks per wave side
@@ -480,6 +489,33 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
             acc1[x][y] = (typename M::acc){0, 0, 0, 0};
             acc2[x][y] = (typename M::acc){0, 0, 0, 0};
         }
+    // FLUSH: the FP64 sums of the finished Q panels, in the FP32 result layout.
+    constexpr int NF = FLUSH ? NB : 1;
+    v4d w1[NF][NF], w2[NF][NF];
+#pragma unroll
+    for (int x = 0; x < NF; ++x)
+#pragma unroll
+        for (int y = 0; y < NF; ++y) {
+            w1[x][y] = (v4d){0, 0, 0, 0};
+            w2[x][y] = (v4d){0, 0, 0, 0};
+        }
+    auto flush = [&]() {
+        if constexpr (FLUSH) {
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        w1[x][y][v] += (double)acc1[x][y][v];
+                        w2[x][y][v] += (double)acc2[x][y][v];
+                    }
+                    acc1[x][y] = (typename M::acc){0, 0, 0, 0};
+                    acc2[x][y] = (typename M::acc){0, 0, 0, 0};
+                }
+        }
+    };
+    const int flush_every = kMixedPanel / sq.step;  // loop iterations per 64 Q rows
 
     // Fragments: the lane's KPL consecutive k values, read as 16-byte chunks.
     auto ldfrag = [&](int buf, int op, int row) -> frag_t {
@@ -539,26 +575,46 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
             compute(cur, std::true_type{}, p * sq.step);
         else
             compute(cur, std::false_type{}, p * sq.step);
+        if constexpr (FLUSH)
+            if ((p + 1) % flush_every == 0 || p + 1 == sq.iters) flush();
         if (p + 1 < sq.iters) lstore(cur ^ 1);
         __syncthreads();
     }
-    if constexpr (RECT) {
+    if constexpr (FLUSH) {
+        if constexpr (RECT) {
 #pragma unroll
-        for (int x = 0; x < NB; ++x)
+            for (int x = 0; x < NB; ++x)
 #pragma unroll
-            for (int y = 0; y < NB; ++y) acc1[x][y] += acc2[x][y];
-        rect_epilogue<TILE>(a, it, L, acc1, rv, cv, RowOf<T>{});
+                for (int y = 0; y < NB; ++y) w1[x][y] += w2[x][y];
+            rect_epilogue<TILE>(a, it, L, w1, rv, cv, RowOf<T>{});
+            return;
+        }
+        if (sq.single) {
+            static_assert(sizeof(lds) >= sizeof(double) * TW * (TILE + 1),
+                          "LDS holds an FP64 C1 half-tile");
+            derive_c2<TILE, double>(it, L, w1, w2, RowOf<T>{}, (double *)&lds[0][0][0][0]);
+        }
+        energy_epilogue<TILE>(a, it, L, w1, w2, rv, cv, RowOf<T>{}, NoCorr{});
         return;
-    }
-    if (sq.single) {
-        static_assert(sizeof(lds) >= sizeof(T) * TW * (TILE + 1), "LDS holds a C1 half-tile");
-        derive_c2<TILE, T>(it, L, acc1, acc2, RowOf<T>{}, (T *)&lds[0][0][0][0]);
-    }
+    } else {
+        if constexpr (RECT) {
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y) acc1[x][y] += acc2[x][y];
+            rect_epilogue<TILE>(a, it, L, acc1, rv, cv, RowOf<T>{});
+            return;
+        }
+        if (sq.single) {
+            static_assert(sizeof(lds) >= sizeof(T) * TW * (TILE + 1), "LDS holds a C1 half-tile");
+            derive_c2<TILE, T>(it, L, acc1, acc2, RowOf<T>{}, (T *)&lds[0][0][0][0]);
+        }
 
-    if constexpr (ADJ)
-        adjoint_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
-    else
-        energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
+        if constexpr (ADJ)
+            adjoint_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
+        else
+            energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -74,8 +74,9 @@ fh_status opposite_core(State &s, const fh_spin_channel &ca, const fh_spin_chann
     fh_status st = reserve(s, req);
     if (st != FH_OK) return st;
     if (prec != Prec::F64 && !i8) {
-        st = range_check(s, ba, in_f32, naux * ca.nocc * ca.nvir, naux, stream);
-        if (st == FH_OK) st = range_check(s, bb, in_f32, naux * cb.nocc * cb.nvir, naux, stream);
+        const size_t depth = prec == Prec::MIXED ? std::min(naux, (size_t)kMixedPanel) : naux;
+        st = range_check(s, ba, in_f32, naux * ca.nocc * ca.nvir, depth, stream);
+        if (st == FH_OK) st = range_check(s, bb, in_f32, naux * cb.nocc * cb.nvir, depth, stream);
         if (st != FH_OK) return st;
     }
     std::vector<double> e;
@@ -276,8 +277,9 @@ static fh_status opposite_entry(const fh_spin_channel *alpha, const fh_spin_chan
     if (!energy) return fail(FH_LAYOUT, "null output pointer");
     Prec prec;
     if (!to_prec(precision, &prec)) return fail(FH_LAYOUT, "unknown precision %d", (int)precision);
-    if (prec != Prec::F64 && prec != Prec::F32 && prec != Prec::I8X4)
-        return fail(FH_UNSUPPORTED, "the opposite-spin kernel runs in FP64, FP32 or INT8X4");
+    if (prec != Prec::F64 && prec != Prec::F32 && prec != Prec::MIXED && prec != Prec::I8X4)
+        return fail(FH_UNSUPPORTED,
+                    "the opposite-spin kernel runs in FP64, FP32, MIXED or INT8X4");
     if (b_dtype != FH_DT_F64 && b_dtype != FH_DT_F32)
         return fail(FH_LAYOUT, "unknown b_dtype %d", (int)b_dtype);
     bool ea = false, eb = false;

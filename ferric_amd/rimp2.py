@@ -32,7 +32,8 @@ __all__ = [
     "SpinComponents", "Precision", "active_occ", "spin_components_from_b_ov",
     "spin_components_from_b_ov_kappa", "spin_components_from_b_ov_kappa_f64",
     "try_spin_components_on_device", "spin_components", "ScsMp2Config", "scs_mp2",
-    "settings_precision", "energy_adjoint",
+    "settings_precision", "energy_adjoint", "mixed_energy_operator", "mixed_ok",
+    "spin_components_fitted",
 ]
 
 
@@ -57,7 +58,7 @@ def active_occ(nocc_total: int, frozen_core: int) -> int:
 
 _PREC_NAMES = {
     "f64": Precision.FP64, "fp64": Precision.FP64,
-    "f32": Precision.FP32, "fp32": Precision.FP32, "mixed": Precision.FP32,
+    "f32": Precision.FP32, "fp32": Precision.FP32, "mixed": Precision.MIXED,
     "bf16x3": Precision.BF16X3, "bf16": Precision.BF16X3,
     "fp8": Precision.FP8_E4M3, "fp8_e4m3": Precision.FP8_E4M3,
     "int8x4": Precision.INT8X4, "int8": Precision.INT8X4,
@@ -69,6 +70,8 @@ def settings_precision() -> Precision:
 
     ``FERRIC_GPU_PRECISION`` (f64 | mixed | fp32 | bf16x3 | fp8 | int8x4), the reference's
     precision knob (crates/ferric-core/src/gpu/precision.rs:283); default f64.
+    ``mixed`` is the reference's own mixed arithmetic (FP32 over 64-row Q panels
+    summed in FP64, Precision.MIXED); the other reduced modes are this library's.
     A malformed value is an error, as with the reference's ConfigVar.
     """
     v = os.environ.get("FERRIC_GPU_PRECISION", "f64").strip().lower()
@@ -230,6 +233,47 @@ def spin_components_from_g(g, eps, nocc: int, nvir: int, first_occ: int = 0,
                                       nocc_total, ctypes.byref(out))
     check(st, "fh_rimp2_energy_from_g")
     return SpinComponents(out.e_os, out.e_ss, out.e_total)
+
+
+# Operator kinds of ferric_integrals::operator::OperatorKind that reach the
+# RI-MP2 energy (crates/ferric-integrals/src/operator.rs:5-40), by name.
+MIXED_ENERGY_OPERATORS = frozenset({"coulomb", "erfc_coulomb", "terfc"})
+OPERATOR_KINDS = MIXED_ENERGY_OPERATORS | frozenset(
+    {"erf_coulomb", "terf", "yukawa", "slater_geminal", "cgtg", "gaussian", "delta"})
+
+
+def mixed_energy_operator(kind: str = "coulomb", composite: bool = False) -> bool:
+    """rimp2.rs:1293-1300: the mixed kernel's error map was measured for the
+    primitive Coulomb, erfc and terfc operators only; composite (fitted)
+    operators and every other kind keep the device in FP64."""
+    k = str(kind).strip().lower()
+    if k not in OPERATOR_KINDS:
+        raise LayoutError(f"unknown operator kind {kind!r}; one of {sorted(OPERATOR_KINDS)}")
+    return (not composite) and k in MIXED_ENERGY_OPERATORS
+
+
+def mixed_ok(kappa=None, fit_metric: str = "coulomb", op_kind: str = "coulomb",
+             composite: bool = False) -> bool:
+    """The caller-side rule of ri_mp2_spin_components_fitted (rimp2.rs:1277):
+    reduced precision only without kappa regularisation, with the Coulomb fit
+    metric (omega-RI has no measured error map) and a mixed-energy operator."""
+    fm = str(fit_metric).strip().lower()
+    if fm not in ("coulomb", "erfc"):
+        raise LayoutError(f"unknown fit metric {fit_metric!r}; expected 'coulomb' or 'erfc'")
+    return kappa is None and fm == "coulomb" and mixed_energy_operator(op_kind, composite)
+
+
+def spin_components_fitted(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa=None,
+                           fit_metric: str = "coulomb", op_kind: str = "coulomb",
+                           composite: bool = False) -> SpinComponents:
+    """The pair-energy call of ri_mp2_spin_components_fitted (rimp2.rs:1266-1281):
+    kappa validated (finite and > 0), ``mixed_ok`` from the operator and fit
+    metric, then dispatch_spin_components."""
+    if kappa is not None and not (math.isfinite(kappa) and kappa > 0.0):
+        raise LayoutError(f"RiMp2Config.kappa must be finite and > 0 (got {kappa}); "
+                          "use None for plain MP2")
+    ok = mixed_ok(kappa, fit_metric, op_kind, composite)
+    return _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, ok)
 
 
 def _dispatch(b_ov, eps, nocc, nvir, first_occ, nocc_total, kappa, mixed_ok) -> SpinComponents:

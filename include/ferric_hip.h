@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 9
+#define FERRIC_HIP_ABI_VERSION 10
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -59,7 +59,14 @@ typedef enum fh_status {
 /* Arithmetic of the (ia|jb) contraction. The orbital-energy denominators and
  * the pair-energy sums are FP64 (compensated) in every mode.
  *   FH_FP64     FP64 MFMA, FP64 accumulation (the plain method)
- *   FH_FP32     FP32 operands, FP32 MFMA, FP32 accumulation
+ *   FH_FP32     FP32 operands, FP32 MFMA, FP32 accumulation over the whole Q range
+ *   FH_MIXED    FP32 operands (RN from the caller's values), FP32 MFMA over Q
+ *               panels of at most 64 rows, each panel's sums added into FP64
+ *               accumulators: the reference's Precision::Mixed arithmetic
+ *               (f32 storage, b = 64 f32 panel products, f64 panel sum;
+ *               crates/ferric-core/src/gpu/mixed.rs:70,122-180); refused
+ *               (FH_RANGE) when max|B| > sqrt(FLT_MAX / 64), as the reference's
+ *               check_f32_arithmetic_range with b = 64
  *   FH_BF16X3   BF16 hi/lo split, hi*hi + hi*lo + lo*hi, FP32 accumulation
  *   FH_FP8_E4M3 OCP E4M3 operands with one power-of-two scale per
  *               (64-row Q block, occupied i), FP32 accumulation
@@ -70,13 +77,14 @@ typedef enum fh_status {
  *               the row's power-of-two range) and the dropped levels depart
  *               from FP64. naux up to 32832 (FH_RANGE beyond).
  * The reference's Precision::{F64, Mixed} (crates/ferric-core/src/gpu/precision.rs)
- * map to FH_FP64 and FH_FP32. */
+ * map to FH_FP64 and FH_MIXED. */
 typedef enum fh_precision {
     FH_FP64 = 0,
     FH_FP32 = 1,
     FH_BF16X3 = 2,
     FH_FP8_E4M3 = 3,
     FH_INT8X4 = 4,
+    FH_MIXED = 5,
 } fh_precision;
 
 /* Element type of a device-resident b_ov. */
@@ -114,6 +122,7 @@ typedef struct fh_stats {
     uint64_t fail_unsupported;
     uint64_t pool_peak_bytes;  /* high-water mark of the library's workspace */
     uint64_t int8x4_calls;
+    uint64_t mixed_calls;
 } fh_stats;
 
 /* One local-MP2 pair domain for fh_rimp2_batched_dev: its B (row-major

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.fh_abi_version() == 9
+    assert L.fh_abi_version() == 10
 
 
 def test_enum_values_match_header():
@@ -45,7 +45,7 @@ def test_enum_values_match_header():
 
 def test_struct_layouts():
     assert ctypes.sizeof(_lib.SpinComponentsC) == 24
-    assert ctypes.sizeof(_lib.StatsC) == 18 * 8
+    assert ctypes.sizeof(_lib.StatsC) == 19 * 8
     assert ctypes.sizeof(_lib.DomainC) == 32
     assert ctypes.sizeof(_lib.TimingC) == 48
     assert ctypes.sizeof(_lib.PnoDiagnosticsC) == 16

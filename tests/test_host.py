@@ -21,13 +21,31 @@ def test_active_occ():
 def test_settings_precision(monkeypatch):
     monkeypatch.delenv("FERRIC_GPU_PRECISION", raising=False)
     assert rimp2.settings_precision() == Precision.FP64
-    for v, p in (("mixed", Precision.FP32), ("BF16X3", Precision.BF16X3),
+    for v, p in (("mixed", Precision.MIXED), ("fp32", Precision.FP32), ("BF16X3", Precision.BF16X3),
                  ("fp8", Precision.FP8_E4M3), ("f64", Precision.FP64)):
         monkeypatch.setenv("FERRIC_GPU_PRECISION", v)
         assert rimp2.settings_precision() == p
     monkeypatch.setenv("FERRIC_GPU_PRECISION", "half")
     with pytest.raises(LayoutError):
         rimp2.settings_precision()
+
+
+def test_mixed_ok_rule():
+    """rimp2.rs:1277 and :1293-1300: reduced precision only for a primitive
+    Coulomb, erfc or terfc operator, the Coulomb fit metric and no kappa."""
+    assert rimp2.mixed_ok()
+    for op in ("coulomb", "erfc_coulomb", "terfc"):
+        assert rimp2.mixed_energy_operator(op)
+        assert not rimp2.mixed_energy_operator(op, composite=True)
+    for op in ("erf_coulomb", "terf", "yukawa", "slater_geminal", "cgtg"):
+        assert not rimp2.mixed_energy_operator(op)
+        assert not rimp2.mixed_ok(op_kind=op)
+    assert not rimp2.mixed_ok(kappa=1.3)
+    assert not rimp2.mixed_ok(fit_metric="erfc")
+    with pytest.raises(LayoutError):
+        rimp2.mixed_ok(op_kind="coloumb")
+    with pytest.raises(LayoutError):
+        rimp2.mixed_ok(fit_metric="gaussian")
 
 
 def test_scs():
