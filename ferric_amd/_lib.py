@@ -137,6 +137,9 @@ SIGNATURES = {
     "fh_rimp2_energy_dev": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz,
                                      _c_sz, _c_double, _c_int, _c_vp,
                                      ctypes.POINTER(SpinComponentsC), _c_dp]),
+    "fh_rimp2_energy_dev_noalloc": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz,
+                                             _c_sz, _c_sz, _c_double, _c_int, _c_vp,
+                                             ctypes.POINTER(SpinComponentsC), _c_dp]),
     "fh_rimp2_energy_adjoint": (_c_int, [_c_dp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz,
                                          _c_sz, _c_double, _c_double, _c_int,
                                          ctypes.POINTER(SpinComponentsC), _c_dp, _c_dp]),

@@ -125,6 +125,22 @@ size_t State::held_bytes() const {
 // before any transfer; a call whose total would exceed the ceiling is refused
 // without allocating anything.
 fh_status reserve(State &s, const std::vector<std::pair<int, size_t>> &req) {
+    if (s.arena) {
+        // Fixed-arena mode: 256-byte aligned slices in request order, no
+        // allocation, no pool accounting.
+        for (const auto &r : req) {
+            const size_t off = (s.arena_used + 255) / 256 * 256;
+            if (off > s.arena_bytes || r.second > s.arena_bytes - off)
+                return fail(FH_POOL_FULL,
+                            "the call needs more than the fixed %zu-byte scratch (%zu bytes "
+                            "placed, %zu more requested)",
+                            s.arena_bytes, off, r.second);
+            s.slots[r.first].ptr = s.arena + off;
+            s.slots[r.first].bytes = r.second;
+            s.arena_used = off + r.second;
+        }
+        return FH_OK;
+    }
     size_t limit = 0;
     fh_status st = resolve_pool_limit(s, &limit);
     if (st != FH_OK) return st;

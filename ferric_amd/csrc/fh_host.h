@@ -74,6 +74,10 @@ struct State {
     int timing = -1;  // -1: not yet read from the environment
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     fh_timing last_timing{};
+    // Fixed arena (fh_rimp2_energy_dev_noalloc): while set, reserve() carves
+    // the requested slots out of it in request order and never allocates.
+    char *arena = nullptr;
+    size_t arena_bytes = 0, arena_used = 0;
     size_t held_bytes() const;
 };
 

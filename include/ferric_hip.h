@@ -210,6 +210,27 @@ fh_status fh_rimp2_energy_dev(const void *b_dev, fh_dtype b_dtype, size_t naux, 
                               size_t nocc_total, double kappa, fh_precision precision,
                               void *stream, fh_spin_components *out, double *row_partials);
 
+/* Bytes of the fixed device workspace of fh_rimp2_energy_dev_noalloc
+ * (128 MiB per device, held from the library's load until the process ends). */
+#define FH_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+
+/* fh_rimp2_energy_dev with no device allocation: the same arguments, kernels
+ * and results (bit for bit), but the workspace (packed operand, per-item and
+ * per-row partials, orbital energies, result) is carved from a fixed
+ * FH_NOALLOC_SCRATCH_BYTES device array instead of the library's pool; a call
+ * whose workspace does not fit is FH_POOL_FULL before any device work. No
+ * counterpart in the reference: this entry exists for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). All calls on one device share the array:
+ * calls that overlap on one device race (the library serialises its own
+ * calls; do not run this entry concurrently from several processes or
+ * streams of one device). FP64 at 41/209/960 needs about 71 MB. */
+fh_status fh_rimp2_energy_dev_noalloc(const void *b_dev, fh_dtype b_dtype, size_t naux,
+                                      size_t nocc, size_t nvir, const double *eps, size_t n_eps,
+                                      size_t first_occ, size_t nocc_total, double kappa,
+                                      fh_precision precision, void *stream,
+                                      fh_spin_components *out, double *row_partials);
+
 /* ---- Adjoint ----------------------------------------------------------- */
 
 /* E = c_os*E_os + c_ss*E_ss (SCS surface, crates/ferric-mp2/src/scs.rs:93-114)
