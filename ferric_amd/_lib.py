@@ -140,6 +140,12 @@ SIGNATURES = {
     "fh_rimp2_energy_dev_noalloc": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz,
                                              _c_sz, _c_sz, _c_double, _c_int, _c_vp,
                                              ctypes.POINTER(SpinComponentsC), _c_dp]),
+    "fh_rimp2_mp2_density": (_c_int, [_c_dp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz, _c_sz,
+                                      _c_double, _c_double, ctypes.POINTER(SpinComponentsC),
+                                      _c_dp, _c_dp]),
+    "fh_rimp2_mp2_density_dev": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz,
+                                          _c_sz, _c_sz, _c_double, _c_double, _c_vp,
+                                          ctypes.POINTER(SpinComponentsC), _c_dp, _c_dp]),
     "fh_rimp2_energy_adjoint": (_c_int, [_c_dp, _c_sz, _c_sz, _c_sz, _c_dp, _c_sz, _c_sz,
                                          _c_sz, _c_double, _c_double, _c_int,
                                          ctypes.POINTER(SpinComponentsC), _c_dp, _c_dp]),
@@ -194,7 +200,7 @@ SIGNATURES = {
 }
 
 _lib = None
-ABI_VERSION = 10  # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
+ABI_VERSION = 11  # FERRIC_HIP_ABI_VERSION of include/ferric_hip.h
 
 
 class LibraryNotBuilt(ImportError):

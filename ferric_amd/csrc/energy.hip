@@ -55,10 +55,11 @@ __device__ __forceinline__ pair::Item decode_item(const EnergyArgs &a) {
     return it;
 }
 
-template <typename T, int TILE, int KP, bool ADJ, bool SIGNED = false, bool FLUSH = false>
+template <typename T, int TILE, int KP, bool ADJ, bool SIGNED = false, bool FLUSH = false,
+          bool AMP = false>
 __global__ __launch_bounds__(256) void pair_k4_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::K4Lds<T, TILE, KP> lds;
-    pair::pair_k4_body<T, TILE, KP, ADJ, false, SIGNED, FLUSH>(a, decode_item(a), lds);
+    pair::pair_k4_body<T, TILE, KP, ADJ, false, SIGNED, FLUSH, AMP>(a, decode_item(a), lds);
 }
 
 // Opposite-spin rectangle: item (i, J, tA, tB), J and tB on the beta side;
@@ -141,6 +142,13 @@ hipError_t launch_items(K kern, const EnergyArgs &a, hipStream_t st) {
 hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
     if (a.nitems <= 0 || a.nitems > 0x7fffffffLL) return hipErrorInvalidValue;
     hipError_t e = hipErrorInvalidValue;
+    if (a.tamp) {  // MP2 densities: FP64 energy that also stores the amplitudes
+        if (a.prec != Prec::F64 || a.q_neg < a.naux_pad) return hipErrorInvalidValue;
+        if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 16, false, false, false, true>, a, st);
+        else if (a.tile == 64) e = launch_items(pair_k4_kernel<double, 64, 16, false, false, false, true>, a, st);
+        else if (a.tile == 32) e = launch_items(pair_k4_kernel<double, 32, 16, false, false, false, true>, a, st);
+        return e;
+    }
     if (a.q_neg < a.naux_pad) {  // signed factor: FP64 / FP32 only
         if (a.prec == Prec::F64) {
             if (a.tile == 128) e = launch_items(pair_k4_kernel<double, 128, 16, false, true>, a, st);

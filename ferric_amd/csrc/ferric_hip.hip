@@ -292,7 +292,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
                       size_t nvir, const double *eps, size_t first_occ, size_t nocc_total,
                       double kappa, Prec prec, hipStream_t stream, fh_spin_components *out,
                       double *row_partials, const EnergyPlan *forced, const void *neg_dev,
-                      size_t naux_neg) {
+                      size_t naux_neg, double *amp_out) {
     const bool in_f32 = b_dtype == FH_DT_F32;
     EnergyPlan pl = forced ? *forced : plan_energy(nocc, nvir, naux, prec);
     // A signed factor stacks B- after B+ along Q, each segment padded to the
@@ -419,6 +419,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     ea.row_partials = (double *)s.slots[kSlotRows].ptr;
     ea.result = (double *)s.slots[kSlotResult].ptr;
     ea.q_neg = q_neg;
+    ea.tamp = amp_out;
     mark(s, 1, stream);
     he = launch_energy(ea, stream);
     if (he != hipSuccess) {

@@ -45,6 +45,9 @@ enum SlotId {
     kSlotFitMeta,    // domain fits: domain index lists, sizes, pairs, Cholesky info
     kSlotGPairs,     // energies from explicit (ia|jb) blocks: the pair list
     kSlotRexpB,      // INT8X4 opposite spin: row exponents of the beta rows
+    kSlotDensAmp,    // MP2 densities: amplitude blocks of every ordered occupied pair
+    kSlotDensPart,   // MP2 densities: split-K partial tiles
+    kSlotDensOut,    // MP2 densities: P_vv (padded) then P_oo
     kNumSlots
 };
 
@@ -122,7 +125,8 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
                       size_t nvir, const double *eps, size_t first_occ, size_t nocc_total,
                       double kappa, Prec prec, hipStream_t stream, fh_spin_components *out,
                       double *row_partials, const EnergyPlan *forced,
-                      const void *neg_dev = nullptr, size_t naux_neg = 0);
+                      const void *neg_dev = nullptr, size_t naux_neg = 0,
+                      double *amp_out = nullptr);
 void commit(State &s, Prec prec, bool adjoint, bool batched);
 bool to_prec(int p, Prec *out);
 void release_graphs(State &s);

@@ -156,6 +156,9 @@ struct EnergyArgs {
     int nocc_b = 0, nvir_b = 0, nvir_pad_b = 0, ntile_b = 0;
     const double *eps_occ_b = nullptr, *eps_vir_b = nullptr;
     const int *rexp_b = nullptr;  // I8X4: row exponents of the beta rows
+    // MP2 densities (FP64): the energy kernel also stores t = g / D of every
+    // ordered occupied pair, tamp[((i*nocc + j)*nvir_pad + a)*nvir_pad + b].
+    double *tamp = nullptr;
 };
 
 hipError_t launch_energy(const EnergyArgs &a, hipStream_t st);
@@ -193,5 +196,27 @@ hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_rect_reduce(const EnergyArgs &a, hipStream_t st);
 // Row reductions and the ascending fold into a.result.
 hipError_t launch_energy_reduce(const EnergyArgs &a, hipStream_t st);
+
+// Split-K FP64 MFMA product of the MP2 densities from stored amplitudes:
+//   out[m][n] = scale * sum_k A[m][k] (cd B1[n][k] - ce B2[n][k])
+// with the rows of A, B1, B2 addressed per mode:
+//   DENS_VV (m, n virtual; k = (q = i*nocc + j, c)):  A = t_ij[m][c],
+//            B1 = t_ij[n][c], B2 = t_ji[n][c]          (P_ab, oo_rimp2.rs:1192-1300)
+//   DENS_OO (m, n occupied; k = (kk, f = a*nvir_pad + b)): A = t_m,kk[f],
+//            B1 = t_n,kk[f], B2 = t_kk,n[f]            (P_ij, scale -1)
+enum class DensMode { VV = 0, OO = 1 };
+struct DensArgs {
+    DensMode mode;
+    const double *tamp;   // ordered-pair amplitude blocks
+    int nocc, nvir_pad;
+    int mdim;             // M = N: nvir_pad (VV) or nocc (OO)
+    int ntile;            // 64-wide output tiles per side
+    int64_t kpanels;      // 16-deep K panels in total
+    int splits;           // K slices per output tile
+    double cd, ce, scale;
+    double *partial;      // (ntile^2 * splits) tiles of 64 x 64
+    double *out;          // (mdim, mdim) row-major
+};
+hipError_t launch_density(const DensArgs &d, hipStream_t st);
 
 }  // namespace fh

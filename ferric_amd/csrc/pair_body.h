@@ -275,6 +275,55 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
     }
 }
 
+// Amplitude store (the MP2 densities): t = g / D of the item's elements into
+// the ordered-pair amplitude blocks, a.tamp[((p*nocc + q)*nvir_pad + x)*nvir_pad + y]
+// = t_pq^xy for every ordered occupied pair (p, q). C1 gives t_ij^ab, stored as
+// (i, j, a, b) and as its mirror (j, i, b, a); C2 gives t_ij^ba, stored as
+// (i, j, b, a) and (j, i, a, b) (on a diagonal tile pair C2 = C1^T, whose
+// elements other lanes store). Elements past nvir are stored as zeros, so the
+// padded blocks read back whole. Every element has exactly one writer.
+template <int TILE, typename Acc, typename RowOfT>
+__device__ __forceinline__ void amp_store(const EnergyArgs &a, const Item &it, const Lane &L,
+                                          Acc (&acc1)[TILE / 32][TILE / 32],
+                                          Acc (&acc2)[TILE / 32][TILE / 32], const bool *rv,
+                                          const bool *cv, RowOfT row_of) {
+    constexpr int TW = TILE / 2, NB = TW / 16;
+    const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
+    const int64_t vp = a.nvir_pad;
+    double *bij = a.tamp + ((int64_t)it.i * a.nocc + it.j) * vp * vp;
+    double *bji = a.tamp + ((int64_t)it.j * a.nocc + it.i) * vp * vp;
+    const bool diag_tile = it.ta == it.tb;
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            if (!(rv[x] && cv[y])) continue;
+            const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
+            const bool bok = bb < a.nvir;
+            const double eb = bok ? a.eps_vir[bb] : 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
+                double t1 = 0.0, t2 = 0.0;
+                if (bok && aa < a.nvir) {
+                    const double inv = 1.0 / (eij - a.eps_vir[aa] - eb);
+                    t1 = acc1[x][y][v] * inv;
+                    t2 = acc2[x][y][v] * inv;
+                }
+                bij[aa * vp + bb] = t1;
+                if (!diag_tile) bij[bb * vp + aa] = t2;
+                // For i == j the mirror block is the block itself: each of its
+                // elements has exactly one writer above (on a diagonal tile
+                // pair the lane of (b, a) writes t^ba, which need not equal
+                // this lane's t^ab to the last bit).
+                if (it.i != it.j) {
+                    bji[bb * vp + aa] = t1;
+                    if (!diag_tile) bji[aa * vp + bb] = t2;
+                }
+            }
+        }
+}
+
 struct NoCorr {
     __device__ double operator()(int, int) const { return 0.0; }
 };
@@ -407,11 +456,13 @@ using K4Lds = T[2][4][TILE][KP + 16 / sizeof(T)];
 // whole 64-row panel, a single-product item's (even / odd 16-row panels) 32
 // rows of it.
 template <typename T, int TILE, int KP, bool ADJ, bool RECT = false, bool SIGNED = false,
-          bool FLUSH = false>
+          bool FLUSH = false, bool AMP = false>
 __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it,
                                              K4Lds<T, TILE, KP> &lds) {
     static_assert(!FLUSH || (std::is_same<T, float>::value && !ADJ && !SIGNED),
                   "the FP64 flush is an FP32 energy mode");
+    static_assert(!AMP || (std::is_same<T, double>::value && !ADJ && !RECT && !SIGNED && !FLUSH),
+                  "the amplitude store is an FP64 energy mode");
     static_assert(kMixedPanel % (2 * KP) == 0, "flushes fall on whole loop iterations");
     using M = Mfma4<T>;
     constexpr int TW = TILE / 2;          // wave sub-tile side (2x2 waves)
@@ -610,10 +661,12 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
             derive_c2<TILE, T>(it, L, acc1, acc2, RowOf<T>{}, (T *)&lds[0][0][0][0]);
         }
 
-        if constexpr (ADJ)
+        if constexpr (ADJ) {
             adjoint_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
-        else
+        } else {
+            if constexpr (AMP) amp_store<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{});
             energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<T>{}, NoCorr{});
+        }
     }
 }
 

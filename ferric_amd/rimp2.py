@@ -33,7 +33,7 @@ __all__ = [
     "spin_components_from_b_ov_kappa", "spin_components_from_b_ov_kappa_f64",
     "try_spin_components_on_device", "spin_components", "ScsMp2Config", "scs_mp2",
     "settings_precision", "energy_adjoint", "mixed_energy_operator", "mixed_ok",
-    "spin_components_fitted",
+    "spin_components_fitted", "mp2_density",
 ]
 
 
@@ -201,6 +201,53 @@ def energy_adjoint(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
         check(st, "fh_rimp2_energy_adjoint")
     sc = SpinComponents(out.e_os, out.e_ss, out.e_total)
     return sc, c_os * sc.e_os + c_ss * sc.e_ss, gamma, deps
+
+
+def mp2_density(b_ov, eps, nocc: int, nvir: int, first_occ: int = 0,
+                nocc_total: int | None = None, c_os: float = 1.0, c_ss: float = 1.0):
+    """The MP2 one-particle densities on the device: ``(SpinComponents, P_oo,
+    P_vv)`` with P_oo (nocc, nocc) and P_vv (nvir, nvir) over the active
+    orbitals, build_mp2_density (crates/ferric-mp2/src/oo_rimp2.rs:1192-1300)
+    for (c_os, c_ss) = (1, 1), its spin-scaled form (rimp2.rs:1917-2052)
+    otherwise. ``b_ov``: host numpy array or contiguous CUDA tensor."""
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    p_oo = np.zeros((nocc, nocc))
+    p_vv = np.zeros((nvir, nvir))
+    out = _lib.SpinComponentsC()
+    L = lib()
+    if _is_torch_cuda(b_ov):
+        import torch
+
+        if b_ov.dtype not in (torch.float64, torch.float32):
+            raise LayoutError(f"b_ov dtype {b_ov.dtype} is not float64/float32")
+        if b_ov.dim() != 2 or not b_ov.is_contiguous():
+            raise LayoutError("b_ov must be a contiguous 2-D (naux, nocc*nvir) tensor")
+        naux, width = b_ov.shape
+        if width != nocc * nvir:
+            raise LayoutError(f"b_ov is {naux}x{width} but nocc*nvir = {nocc * nvir}")
+        dt = _lib.DType.F64 if b_ov.dtype == torch.float64 else _lib.DType.F32
+        stream = torch.cuda.current_stream(b_ov.device).cuda_stream
+        st = L.fh_rimp2_mp2_density_dev(
+            ctypes.c_void_p(b_ov.data_ptr()), int(dt), naux, nocc, nvir,
+            e.ctypes.data_as(_lib._c_dp), e.size, first_occ, nocc_total, float(c_os), float(c_ss),
+            ctypes.c_void_p(stream), ctypes.byref(out), p_oo.ctypes.data_as(_lib._c_dp),
+            p_vv.ctypes.data_as(_lib._c_dp))
+        check(st, "fh_rimp2_mp2_density_dev")
+    else:
+        b = np.ascontiguousarray(np.asarray(b_ov, dtype=np.float64))
+        if b.ndim != 2:
+            raise LayoutError("b_ov must be 2-D (naux, nocc*nvir)")
+        naux, width = b.shape
+        if width != nocc * nvir:
+            raise LayoutError(f"b_ov is {naux}x{width} but nocc*nvir = {nocc * nvir}")
+        st = L.fh_rimp2_mp2_density(
+            b.ctypes.data_as(_lib._c_dp), naux, nocc, nvir, e.ctypes.data_as(_lib._c_dp), e.size,
+            first_occ, nocc_total, float(c_os), float(c_ss), ctypes.byref(out),
+            p_oo.ctypes.data_as(_lib._c_dp), p_vv.ctypes.data_as(_lib._c_dp))
+        check(st, "fh_rimp2_mp2_density")
+    return SpinComponents(out.e_os, out.e_ss, out.e_total), p_oo, p_vv
 
 
 def spin_components_from_g(g, eps, nocc: int, nvir: int, first_occ: int = 0,

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FERRIC_HIP_ABI_VERSION 10
+#define FERRIC_HIP_ABI_VERSION 11
 
 /* GpuError classes (crates/ferric-core/src/gpu/device.rs:16-36). */
 typedef enum fh_status {
@@ -259,6 +259,31 @@ fh_status fh_rimp2_energy_adjoint_dev(const void *b_dev, fh_dtype b_dtype, size_
                                       size_t first_occ, size_t nocc_total, double c_os,
                                       double c_ss, fh_precision precision, void *stream,
                                       fh_spin_components *out, double *gamma_dev, double *deps);
+
+/* ---- MP2 one-particle densities ----------------------------------------- */
+
+/* The MP2 densities of E = c_os E_os + c_ss E_ss over the active orbitals:
+ *   p_oo[i*nocc + j] = P_ij = - sum_k sum_ab t_ik^ab T~_jk^ab
+ *   p_vv[a*nvir + b] = P_ab =   sum_ij sum_c t_ij^ac T~_ij^bc
+ * with t = (ia|jb) / D and T~ = (c_os + c_ss) t - c_ss t^T: for (1, 1) the
+ * reference's build_mp2_density (crates/ferric-mp2/src/oo_rimp2.rs:1192-1300,
+ * T~ = 2t - t^T), otherwise its spin-scaled c_ss P_MP2 + (c_os - c_ss) P_OS
+ * (apply_spin_scaling / opposite_spin_density, rimp2.rs:1917-2052); host
+ * arrays, row-major, symmetric. diag(P) is what fh_rimp2_energy_adjoint
+ * returns as deps / 2. out receives the plain (E_os, E_ss). FP64 only. The
+ * amplitudes of every ordered occupied pair are held in the workspace, as the
+ * reference holds t2: 8 nocc^2 nvir_pad^2 bytes (nvir_pad = nvir rounded up
+ * to 16; 0.68 GB at 41/209, 66.5 GB at 100/900), refused with FH_POOL_FULL
+ * beyond the pool ceiling. */
+fh_status fh_rimp2_mp2_density(const double *b_ov, size_t naux, size_t nocc, size_t nvir,
+                               const double *eps, size_t n_eps, size_t first_occ,
+                               size_t nocc_total, double c_os, double c_ss,
+                               fh_spin_components *out, double *p_oo, double *p_vv);
+/* The same with a device-resident b_ov (FP64 or FP32) on the given stream. */
+fh_status fh_rimp2_mp2_density_dev(const void *b_dev, fh_dtype b_dtype, size_t naux, size_t nocc,
+                                   size_t nvir, const double *eps, size_t n_eps, size_t first_occ,
+                                   size_t nocc_total, double c_os, double c_ss, void *stream,
+                                   fh_spin_components *out, double *p_oo, double *p_vv);
 
 /* ---- Batched local-MP2 domains ---------------------------------------- */
 

@@ -217,6 +217,34 @@ def mp2_density(t2, nocc, nvir, c_os=1.0, c_ss=1.0):
     return p_oo, p_vv
 
 
+def mp2_density_blas(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, c_os=1.0, c_ss=1.0):
+    """build_mp2_density (oo_rimp2.rs:1192-1300) with the spin scaling of
+    rimp2.rs:1917-2052, as numpy BLAS contractions over the dense amplitudes
+    t[i, a, j, b] = (ia|jb) / D (the C oracle's orc_t2_and_integrals and
+    orc_scaled_density restated for sizes where its plain loops are slow):
+    P_oo = -sum_kab t_ik^ab T~_jk^ab, P_vv = sum_ijc t_ij^ac T~_ij^bc with
+    T~ = (c_os + c_ss) t - c_ss t^T (t^T swapping a and b)."""
+    b = _f64(b_ov)
+    e = _f64(eps)
+    if nocc_total is None:
+        nocc_total = first_occ + nocc
+    eo = e[first_occ:first_occ + nocc]
+    ev = e[nocc_total:nocc_total + nvir]
+    g = (b.T @ b).reshape(nocc, nvir, nocc, nvir)
+    d = eo[:, None, None, None] + eo[None, None, :, None] - ev[None, :, None, None] \
+        - ev[None, None, None, :]
+    t = g / d
+    del g, d
+    tt = (c_os + c_ss) * t - c_ss * t.transpose(0, 3, 2, 1)
+    # P_oo[i, j] = -sum_{k,a,b} t[i,a,k,b] tt[j,a,k,b]
+    p_oo = -(t.reshape(nocc, -1) @ tt.reshape(nocc, -1).T)
+    # P_vv[a, b] = sum_{i,j,c} t[i,a,j,c] tt[i,b,j,c]
+    ta = np.ascontiguousarray(t.transpose(1, 0, 2, 3)).reshape(nvir, -1)
+    tb = np.ascontiguousarray(tt.transpose(1, 0, 2, 3)).reshape(nvir, -1)
+    p_vv = ta @ tb.T
+    return p_oo, p_vv
+
+
 def adjoint(b_ov, eps, nocc, nvir, first_occ=0, nocc_total=None, c_os=1.0, c_ss=1.0):
     """(E_os, E_ss, E, Gamma = dE/dB, dE/deps) through the reference's intermediates."""
     b = _f64(b_ov)

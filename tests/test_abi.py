@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.fh_abi_version() == 10
+    assert L.fh_abi_version() == 11
 
 
 def test_enum_values_match_header():

@@ -379,6 +379,32 @@ def test_adjoint_invariants(oracle_mod, golden):
         assert abs(np.sum(deps)) < 1e-11 * np.sum(np.abs(deps))
 
 
+@pytest.mark.parametrize("scaling", [(1.0, 1.0), (1.2, 1.0 / 3.0), (0.0, 1.0)])
+def test_blas_density_matches_the_loop_oracle(oracle_mod, golden, scaling):
+    """The BLAS restatement of build_mp2_density (used at decane scale) equals
+    the C oracle's plain loops on the golden cases, frozen core included."""
+    for name in ("res_120_7_23_1_3", "res_150_9_31_2_5", "water_shapes", "lcg_120_8_40_2"):
+        c = golden[name]
+        args = (c["b"], c["eps"], c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"])
+        t2, _ = oracle_mod.t2_and_integrals(*args)
+        w_oo, w_vv = oracle_mod.mp2_density(t2, c["nocc"], c["nvir"], *scaling)
+        g_oo, g_vv = oracle_mod.mp2_density_blas(*args, *scaling)
+        assert np.max(np.abs(g_oo - w_oo)) <= 1e-13 * np.max(np.abs(w_oo)), name
+        assert np.max(np.abs(g_vv - w_vv)) <= 1e-13 * np.max(np.abs(w_vv)), name
+
+
+def test_density_diagonal_is_the_orbital_energy_gradient(oracle_mod, golden):
+    """dE/deps_k = 2 P_kk and dE/deps_c = 2 P_cc (the adjoint's closed form) on
+    the densities the BLAS restatement forms, SCS weights."""
+    c = golden["res_150_9_31_2_5"]
+    args = (c["b"], c["eps"], c["nocc"], c["nvir"], c["first_occ"], c["nocc_total"])
+    p_oo, p_vv = oracle_mod.mp2_density_blas(*args, 1.2, 0.4)
+    _, _, _, _, deps = oracle_mod.adjoint(*args, 1.2, 0.4)
+    fc, nt = c["first_occ"], c["nocc_total"]
+    assert np.allclose(2 * np.diag(p_oo), deps[fc:fc + c["nocc"]], rtol=1e-12, atol=0)
+    assert np.allclose(2 * np.diag(p_vv), deps[nt:nt + c["nvir"]], rtol=1e-12, atol=0)
+
+
 def test_density_symmetry_and_trace(oracle_mod, golden):
     """rimp2.rs:3291-3350: P_oo, P_vv symmetric; tr P_oo + tr P_vv = 0."""
     c = golden["res_120_7_23_1_3"]
