@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--variants", default="FP64,FP32,MIXED,BF16X3,FP8_E4M3,INT8X4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-variants", default="FP64,BF16X3,INT8X4")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-dressing", action="store_true")
     ap.add_argument("--no-from-g", action="store_true")
@@ -241,7 +242,7 @@ def main():
         from ferric_amd.rimp2 import energy_adjoint
 
         g64 = None
-        for prec in (Precision.FP64, Precision.BF16X3, Precision.INT8X4):
+        for prec in [Precision[v] for v in args.adjoint_variants.split(",")]:
             energy_adjoint(b, p.eps, p.nocc, p.nvir, precision=prec)   # warm
             torch.cuda.synchronize()
             k = max(1, min(args.steps, 3))
@@ -252,9 +253,9 @@ def main():
             ms = (time.perf_counter() - t0) * 1e3 / k
             adj[prec.name] = {"ms_per_step": ms, "E": e_tot, "tflops_3FE": 3 * F / (ms * 1e-3) / 1e12,
                               "gamma_abs_max": float(gam.abs().max())}
-            if g64 is None:
+            if prec == Precision.FP64:
                 g64 = gam
-            else:
+            elif g64 is not None:
                 adj[prec.name]["dE_vs_fp64"] = e_tot - adj["FP64"]["E"]
                 adj[prec.name]["max_rel_dGamma_vs_fp64"] = float((gam - g64).abs().max()
                                                                  / g64.abs().max())

@@ -21,15 +21,17 @@
 // stream-ordered, so the result is deterministic. T~ is never held for more
 // than one occupied row.
 //
-// INT8X4 (adjoint_core_i8): phase A runs once over every item on the INT8
-// digit-slice kernel (energy_i8.hip), which writes the whole T~ -- symmetric,
-// T~_ji^ba = T~_ij^ab -- as BF16 hi / lo over the padded (i, a) index space
-// (2 (nocc nvir_pad)^2 x 2 bytes: 33 GB at 100/900/3600, held in HBM); the
-// per-row reductions follow; then Gamma = 4 B T~ is one plain GEMM over that
-// space, handed to rocBLAS as the three products hi*hi + hi*lo + lo*hi with
-// FP32 accumulation (rocBLAS atomics off: deterministic) and widened to FP64.
+// INT8X4 (adjoint_core_i8): groups of R occupied rows. Phase A of a group runs
+// on the INT8 digit-slice kernel (energy_i8.hip) over the items of its rows and
+// writes their T~ -- symmetric, T~_ji^ba = T~_ij^ab -- as BF16 hi / lo into two
+// slabs over the padded (i, a) index space: S (the group's rows, every column
+// from the group on) and St (the transposed blocks of the columns past the
+// group), 2 x R vpg x nocc vpg x 4 bytes (5.5 GB at 100/900/3600 with R = 8,
+// against the 64.8 GB amplitudes the reference holds); the per-row reductions
+// follow; then two hand-written BF16x3 GEMMs (bf16x3_nt_kernel, FP32
+// accumulation) add this group's part of Gamma = 4 B T~ into an FP32 gamma,
+// widened to FP64 at the end.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -380,6 +382,133 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g) {
     gemm_store<TILE>(g, t, wr, wc, r, gg, acc, RowOfG<float>{});
 }
 
+// ---- BF16x3 NT GEMM on pre-split operands (the INT8X4 adjoint's Gamma) ------
+// 256 x 256 output tile per workgroup of eight waves (2 along M x 4 along N,
+// each 128 x 64 = 4 x 2 blocks of v_mfma_f32_32x32x16_bf16). At this tile a
+// 32-deep K panel is 64 KB of operands (A and B, hi and lo) for 12.6 MFLOP of
+// MFMA work, about 190 FLOP per byte: the operand stream stays well inside the
+// L2 bandwidth at the BF16 matrix rate. Per panel the workgroup stages the
+// four operand slices into LDS (double-buffered, one barrier per panel); each
+// LDS row holds 32 hi then 32 lo values as eight 16-byte chunks, chunk c at
+// slot c ^ ((row >> 1) & 7): the 16 rows one 16-lane phase of a fragment read
+// touches then cover all 16 four-bank groups (rows 128 bytes apart share a
+// bank group every other row, so the slots of the even and of the odd rows
+// must each be distinct). Per block and 16-deep step: C += lo*hi + hi*lo + hi*hi
+// in FP32. Rows past M or N load zeros and store nothing. Tiles are walked m
+// fastest after the XCD remap.
+//
+// 32x32x16 lane layout (checked on the device, tests/test_gpu_layout.py):
+// lane l = 32 h + c supplies A[row c][k = 8 h .. 8 h + 7] and B[k = 8 h ..][col c];
+// result element j of lane l is C[8 (j / 4) + 4 h + j % 4][c].
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(512) void bf16x3_nt_kernel(Bf16x3Gemm g) {
+    constexpr int T = 256, KP = 32;
+    __shared__ __attribute__((aligned(16))) v4i lds[2][2][T][8];
+    const int mt = (g.M + T - 1) / T, nt = (g.N + T - 1) / T;
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, (int64_t)mt * nt);
+    const int m0 = (int)(tile % mt) * T, n0 = (int)(tile / mt) * T;
+    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+    const int wm = wid / 4, wn = wid % 4, c = lane % 32, h = lane / 32;
+
+    // chunk q = tid + 512 u: operand u >> 2 (A for u < 4, B after), row
+    // (tid >> 3) + 64 (u & 3), piece tid & 7 (pieces 0-3 hi, 4-7 lo, each 8
+    // consecutive k). Row pointers are formed once; rows past M or N read row
+    // 0 of their operand and are zeroed.
+    const int piece = tid & 7;
+    const bool hi_piece = piece < 4;
+    const uint16_t *const a_base = hi_piece ? g.ahi : g.alo;
+    const uint16_t *const b_base = hi_piece ? g.bhi : g.blo;
+    const uint16_t *src[8];
+    bool ok[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int row = (tid >> 3) + 64 * (u & 3);
+        const int grow = (u < 4 ? m0 : n0) + row;
+        ok[u] = grow < (u < 4 ? g.M : g.N);
+        const int64_t r = ok[u] ? grow : 0;
+        src[u] = (u < 4 ? a_base + r * g.lda : b_base + r * g.ldb) + (piece & 3) * 8;
+    }
+    v4i stg[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const v4i v = *(const v4i *)(src[u] + k0);
+            stg[u] = ok[u] ? v : (v4i){0, 0, 0, 0};
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int row = (tid >> 3) + 64 * (u & 3);
+            lds[buf][u >> 2][row][piece ^ ((row >> 1) & 7)] = stg[u];
+        }
+    };
+    v16f acc[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[x][y][j] = 0.0f;
+    auto frag = [&](int buf, int op, int row, int chunk) -> v8bf {
+        return __builtin_bit_cast(v8bf, lds[buf][op][row][chunk ^ ((row >> 1) & 7)]);
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            v8bf ha[4], la[4], hb[2], lb[2];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const int ra = wm * 128 + x * 32 + c;
+                ha[x] = frag(buf, 0, ra, 2 * kk + h);
+                la[x] = frag(buf, 0, ra, 4 + 2 * kk + h);
+            }
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int rb = wn * 64 + y * 32 + c;
+                hb[y] = frag(buf, 1, rb, 2 * kk + h);
+                lb[y] = frag(buf, 1, rb, 4 + 2 * kk + h);
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    v16f a = acc[x][y];
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la[x], hb[y], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha[x], lb[y], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha[x], hb[y], a, 0, 0, 0);
+                    acc[x][y] = a;
+                }
+        }
+    };
+    const int np = g.K / KP;
+    if (np > 0) {
+        gload(0);
+        lstore(0);
+    }
+    __syncthreads();
+    for (int p = 0; p < np; ++p) {
+        const int cur = p & 1;
+        if (p + 1 < np) gload((p + 1) * KP);
+        compute(cur);
+        if (p + 1 < np) lstore(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int n = n0 + wn * 64 + y * 32 + c;
+            if (n >= g.N) continue;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int m = m0 + wm * 128 + x * 32 + 8 * (j / 4) + 4 * h + j % 4;
+                if (m < g.M) g.c[(int64_t)m * g.ldc + n] += g.alpha * acc[x][y][j];
+            }
+        }
+}
+
 // ---- Row reduction of one batch: energy and dE/deps -------------------------
 struct AdjRed {
     int i, nocc, nvir, tile, ntile, ntp, vec_stride;
@@ -571,11 +700,6 @@ void launch_split(const float *x, int64_t nr, int64_t cols, int64_t ld, uint16_t
     hipLaunchKernelGGL(split_bf16_kernel, dim3(blocks), dim3(256), 0, st, x, nr, cols, ld, hi, lo);
 }
 
-fh_status blas_ok(rocblas_status st, const char *what) {
-    if (st == rocblas_status_success) return FH_OK;
-    return fail(FH_KERNEL, "%s: rocBLAS status %d", what, (int)st);
-}
-
 template <typename T>
 void launch_fill(T *p, int64_t n, T v, hipStream_t st) {
     int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
@@ -584,6 +708,14 @@ void launch_fill(T *p, int64_t n, T v, hipStream_t st) {
 }
 
 }  // namespace
+
+hipError_t launch_bf16x3_nt(const Bf16x3Gemm &g, hipStream_t st) {
+    if (g.M <= 0 || g.N <= 0 || g.K < 0 || g.K % 32 != 0) return hipErrorInvalidValue;
+    const int64_t tiles = (int64_t)((g.M + 255) / 256) * ((g.N + 255) / 256);
+    if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bf16x3_nt_kernel, dim3((unsigned)tiles), dim3(512), 0, st, g);
+    return hipGetLastError();
+}
 
 hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st) {
     const int tm = g.m_pad / g.tile, tn = (g.nvir_pad + g.tile - 1) / g.tile;
@@ -627,6 +759,16 @@ static int gemm_tile(Prec prec) {
     return 64;
 }
 
+// Occupied rows per group of the INT8X4 adjoint (FERRIC_I8_ADJ_ROWS overrides).
+static int i8_adjoint_rows(size_t nocc) {
+    int r = 8;
+    if (const char *e = std::getenv("FERRIC_I8_ADJ_ROWS")) {
+        const int v = std::atoi(e);
+        if (v > 0) r = v;
+    }
+    return (int)std::min<size_t>(nocc, (size_t)r);
+}
+
 static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux,
                                  size_t nocc, size_t nvir, const double *eps, size_t n_eps,
                                  size_t first_occ, size_t nocc_total, double c_os, double c_ss,
@@ -638,20 +780,25 @@ static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, 
     if (pl.naux_pad > kI8MaxNauxPad)
         return fail(FH_RANGE, "naux_pad %d beyond the INT8 slices' int32 range (%d)", pl.naux_pad,
                     kI8MaxNauxPad);
-    const size_t rows = nocc * (size_t)pl.nvir_pad;
-    if (rows > (size_t)0x7fffffff)
-        return fail(FH_UNSUPPORTED, "nocc * nvir_pad = %zu beyond 32-bit GEMM sizes", rows);
+    // GEMM index space: virtuals padded to the 32-deep K panels of the Gamma GEMMs.
+    const int vpg = (int)((nvir + 31) / 32 * 32);
+    const size_t ncol = nocc * (size_t)vpg;  // (i, a) columns of B2, S and gamma32
+    if (ncol > (size_t)0x7fffffff || naux > (size_t)0x7fffffff)
+        return fail(FH_UNSUPPORTED, "nocc * vpg = %zu beyond 32-bit GEMM sizes", ncol);
+    const int R = i8_adjoint_rows(nocc);
+    const size_t slab = (size_t)R * vpg * ncol;  // elements of S (and of St)
+    const size_t rows8 = nocc * (size_t)pl.nvir_pad;
     const int vec_stride = 2 * pl.tile + 2;
     const size_t items = (size_t)pl.nitems;
     const int nvb = (int)((nvir + 255) / 256);
     const int nchunks_max = (int)std::min<size_t>(nocc, (size_t)std::max(1, (512 + nvb - 1) / nvb));
     std::vector<std::pair<int, size_t>> req = {
         {kSlotOp0, packed_op_bytes(pl, nocc, prec)},
-        {kSlotScale, sizeof(int) * rows},
-        {kSlotStage2, sizeof(float) * naux * rows},
-        {kSlotB2Split, 2 * sizeof(uint16_t) * naux * rows},
-        {kSlotTSplit, 2 * sizeof(uint16_t) * rows * rows},
-        {kSlotG32, sizeof(float) * naux * rows},
+        {kSlotScale, sizeof(int) * rows8},
+        {kSlotStage2, sizeof(float) * naux * ncol},
+        {kSlotB2Split, 2 * sizeof(uint16_t) * naux * ncol},
+        {kSlotTSplit, 4 * sizeof(uint16_t) * slab},
+        {kSlotG32, sizeof(float) * naux * ncol},
         {kSlotEps, sizeof(double) * (nocc + nvir)},
         {kSlotItems, sizeof(double) * 4 * items},
         {kSlotDeps, sizeof(double) * vec_stride * items},
@@ -695,28 +842,32 @@ static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, 
     if (he != hipSuccess) return fail(FH_KERNEL, "pack kernel: %s", hipGetErrorString(he));
     float *b2 = (float *)s.slots[kSlotStage2].ptr;
     {
-        const int64_t total = (int64_t)naux * (int64_t)rows;
+        const int64_t total = (int64_t)naux * (int64_t)ncol;
         const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 16384);
         if (in_f32)
             hipLaunchKernelGGL((pack_qmajor_kernel<float, float>), dim3(blocks), dim3(256), 0,
-                               stream, (const float *)b_dev, (int)naux, (int)nocc, (int)nvir,
-                               pl.nvir_pad, (int)naux, b2);
+                               stream, (const float *)b_dev, (int)naux, (int)nocc, (int)nvir, vpg,
+                               (int)naux, b2);
         else
             hipLaunchKernelGGL((pack_qmajor_kernel<double, float>), dim3(blocks), dim3(256), 0,
                                stream, (const double *)b_dev, (int)naux, (int)nocc, (int)nvir,
-                               pl.nvir_pad, (int)naux, b2);
+                               vpg, (int)naux, b2);
     }
-    uint16_t *b2hi = (uint16_t *)s.slots[kSlotB2Split].ptr, *b2lo = b2hi + naux * rows;
-    uint16_t *thi = (uint16_t *)s.slots[kSlotTSplit].ptr, *tlo = thi + rows * rows;
+    uint16_t *b2hi = (uint16_t *)s.slots[kSlotB2Split].ptr, *b2lo = b2hi + naux * ncol;
+    uint16_t *shi = (uint16_t *)s.slots[kSlotTSplit].ptr, *slo = shi + slab;
+    uint16_t *sthi = slo + slab, *stlo = sthi + slab;
     float *x32 = (float *)s.slots[kSlotG32].ptr;
-    launch_split(b2, (int64_t)naux, (int64_t)rows, (int64_t)rows, b2hi, b2lo, stream);
+    launch_split(b2, (int64_t)naux, (int64_t)ncol, (int64_t)ncol, b2hi, b2lo, stream);
     double *acc = (double *)s.slots[kSlotAcc].ptr;
     launch_fill<double>(acc, (int64_t)(2 * (nocc + nvir)), 0.0, stream);
+    launch_fill<float>(x32, (int64_t)(naux * ncol), 0.0f, stream);
+    // The slabs' padding columns (nvir_pad <= y < vpg) are never written by
+    // phase A and must read as zero: clear both slabs once.
+    launch_fill<uint64_t>((uint64_t *)shi, (int64_t)(slab), (uint64_t)0, stream);
     he = hipGetLastError();
     if (he != hipSuccess) return fail(FH_KERNEL, "adjoint set-up kernels: %s", hipGetErrorString(he));
     mark(s, 1, stream);
 
-    // Phase A: every item in one launch; T~ into the whole-T~ matrix.
     EnergyArgs ea;
     ea.prec = prec;
     ea.tile = pl.tile;
@@ -735,7 +886,6 @@ static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, 
     ea.kappa = NAN;
     ea.ntile = pl.ntile;
     ea.ntp = pl.ntp;
-    ea.nitems = pl.nitems;
     ea.item_partials = (double *)s.slots[kSlotItems].ptr;
     ea.row_partials = (double *)s.slots[kSlotRows].ptr;
     ea.result = (double *)s.slots[kSlotResult].ptr;
@@ -744,14 +894,13 @@ static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, 
     ea.item_vec = (double *)s.slots[kSlotDeps].ptr;
     ea.vec_stride = vec_stride;
     ea.i_fixed = -1;
-    ea.tfull_hi = thi;
-    ea.tfull_lo = tlo;
-    ea.ld_full = (int64_t)rows;
-    he = launch_adjoint_pairs(ea, stream);
-    if (he != hipSuccess) return fail(FH_KERNEL, "adjoint pair kernel: %s", hipGetErrorString(he));
+    ea.s_hi = shi;
+    ea.s_lo = slo;
+    ea.st_hi = sthi;
+    ea.st_lo = stlo;
+    ea.vpg = vpg;
+    ea.ld_s = (int64_t)ncol;
 
-    // Per-row reductions of the energy and dE/deps (each row's items are the
-    // contiguous pairs (i, j >= i)).
     AdjRed rd;
     rd.nocc = (int)nocc;
     rd.nvir = (int)nvir;
@@ -766,45 +915,69 @@ static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, 
     rd.dvir_c = acc + 2 * nocc + nvir;
     rd.vpart = (double *)s.slots[kSlotRed].ptr;
     rd.ppart = rd.vpart + 2 * (size_t)nchunks_max * nvir;
-    for (size_t i = 0; i < nocc; ++i) {
-        const int64_t first = ((int64_t)i * (int64_t)nocc - (int64_t)i * ((int64_t)i - 1) / 2) * pl.ntp;
-        rd.i = (int)i;
-        rd.item_partials = ea.item_partials + 4 * first;
-        rd.item_vec = ea.item_vec + first * vec_stride;
-        const int npair = (int)(nocc - i);
-        rd.nchunks = std::min(npair, nchunks_max);
-        rd.jc = (npair + rd.nchunks - 1) / rd.nchunks;
-        rd.nchunks = (npair + rd.jc - 1) / rd.jc;
-        hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)(nvb * rd.nchunks + npair)),
-                           dim3(256), 0, stream, rd);
-        hipLaunchKernelGGL(adjoint_fold_kernel, dim3((unsigned)nvb), dim3(256), 0, stream, rd);
-    }
-    he = hipGetLastError();
-    if (he != hipSuccess) return fail(FH_KERNEL, "adjoint reduction: %s", hipGetErrorString(he));
+    auto first_item = [&](int64_t i) {
+        return (i * (int64_t)nocc - i * (i - 1) / 2) * pl.ntp;
+    };
 
-    // Gamma = 4 B T~ (column-major for rocBLAS: X^T (rows x naux) = T~ (rows x
-    // rows, symmetric) x B^T (rows x naux)), three BF16 products.
-    {
-        void *hv = nullptr;
-        st = blas_handle(s, stream, &hv);
-        if (st != FH_OK) return st;
-        const float alpha = 4.0f, zero = 0.0f, one = 1.0f;
-        const uint16_t *tp[3] = {thi, thi, tlo}, *bp[3] = {b2hi, b2lo, b2hi};
-        const int r = (int)rows;
-        for (int p = 0; p < 3; ++p) {
-            st = blas_ok(rocblas_gemm_ex((rocblas_handle)hv, rocblas_operation_none,
-                                         rocblas_operation_none, r, (int)naux, r, &alpha, tp[p],
-                                         rocblas_datatype_bf16_r, r, bp[p], rocblas_datatype_bf16_r,
-                                         r, p == 0 ? &zero : &one, x32, rocblas_datatype_f32_r, r,
-                                         x32, rocblas_datatype_f32_r, r, rocblas_datatype_f32_r,
-                                         rocblas_gemm_algo_standard, 0, 0),
-                         "adjoint gamma GEMM");
-            if (st != FH_OK) return st;
+    // One group of R occupied rows at a time: phase A writes its T~ slabs, the
+    // per-row reductions follow, then
+    //   GEMM 1: gamma[Q, (i, a)] += 4 sum_{(j, b), j >= g0} B[Q, (j, b)] S[(i, a), (j, b)]
+    //   GEMM 2: gamma[Q, (j, b)] += 4 sum_{(i, a) in group} B[Q, (i, a)] St[(j, b), (i, a)],
+    //           j >= g1
+    // (T~ symmetric: every ordered occupied pair enters gamma once).
+    for (int g0 = 0; g0 < (int)nocc; g0 += R) {
+        const int g1 = std::min<int>((int)nocc, g0 + R);
+        ea.g0 = g0;
+        ea.g1 = g1;
+        ea.ld_st = (int64_t)(g1 - g0) * vpg;
+        ea.item0 = first_item(g0);
+        ea.nitems = first_item(g1) - ea.item0;
+        he = launch_adjoint_pairs(ea, stream);
+        if (he != hipSuccess)
+            return fail(FH_KERNEL, "adjoint pair kernel: %s", hipGetErrorString(he));
+        for (int i = g0; i < g1; ++i) {
+            const int64_t first = first_item(i);
+            rd.i = i;
+            rd.item_partials = ea.item_partials + 4 * first;
+            rd.item_vec = ea.item_vec + first * vec_stride;
+            const int npair = (int)nocc - i;
+            rd.nchunks = std::min(npair, nchunks_max);
+            rd.jc = (npair + rd.nchunks - 1) / rd.nchunks;
+            rd.nchunks = (npair + rd.jc - 1) / rd.jc;
+            hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)(nvb * rd.nchunks + npair)),
+                               dim3(256), 0, stream, rd);
+            hipLaunchKernelGGL(adjoint_fold_kernel, dim3((unsigned)nvb), dim3(256), 0, stream, rd);
         }
+        Bf16x3Gemm g;
+        g.ahi = b2hi + (size_t)g0 * vpg;
+        g.alo = b2lo + (size_t)g0 * vpg;
+        g.lda = (int64_t)ncol;
+        g.bhi = shi + (size_t)g0 * vpg;
+        g.blo = slo + (size_t)g0 * vpg;
+        g.ldb = (int64_t)ncol;
+        g.c = x32 + (size_t)g0 * vpg;
+        g.ldc = (int64_t)ncol;
+        g.M = (int)naux;
+        g.N = (g1 - g0) * vpg;
+        g.K = ((int)nocc - g0) * vpg;
+        g.alpha = 4.0f;
+        he = launch_bf16x3_nt(g, stream);
+        if (he == hipSuccess && g1 < (int)nocc) {
+            g.bhi = sthi + (size_t)g1 * vpg * ea.ld_st;
+            g.blo = stlo + (size_t)g1 * vpg * ea.ld_st;
+            g.ldb = ea.ld_st;
+            g.c = x32 + (size_t)g1 * vpg;
+            g.N = ((int)nocc - g1) * vpg;
+            g.K = (g1 - g0) * vpg;
+            he = launch_bf16x3_nt(g, stream);
+        }
+        if (he != hipSuccess) return fail(FH_KERNEL, "adjoint GEMM: %s", hipGetErrorString(he));
+    }
+    {
         const int64_t n = (int64_t)naux * (int64_t)(nocc * nvir);
         const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
         hipLaunchKernelGGL(widen_kernel, dim3(blocks), dim3(256), 0, stream, x32, (int)naux,
-                           (int)nocc, (int)nvir, pl.nvir_pad, gamma_dev);
+                           (int)nocc, (int)nvir, vpg, gamma_dev);
     }
     mark(s, 2, stream);
     {
@@ -816,7 +989,8 @@ static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, 
     he = hipGetLastError();
     if (he != hipSuccess) return fail(FH_KERNEL, "adjoint finish: %s", hipGetErrorString(he));
     mark(s, 3, stream);
-    s.pending.kernel_launches += 2 * nocc + 12;
+    const int ngroups = (int)((nocc + R - 1) / R);
+    s.pending.kernel_launches += 2 * nocc + 3 * ngroups + 8;
     std::vector<double> res(2 + nocc + nvir);
     st = hip_check(hipMemcpyAsync(res.data(), ea.result, sizeof(double) * res.size(),
                                   hipMemcpyDeviceToHost, stream),

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
     __shared__ int rexp_t[4][TILE];
 
     pair::Item it;
-    it.id = pair_interleave((int64_t)blockIdx.x, a.ntp, a.nitems / a.ntp);
+    it.id = a.item0 + pair_interleave((int64_t)blockIdx.x, a.ntp, a.nitems / a.ntp);
     decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
     if (ADJ)  // the adjoint's row reduction reads each item's sums in decode_tile_pair's order
         decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
@@ -547,7 +547,10 @@ hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st) {
 
 hipError_t launch_adjoint_i8x4(const EnergyArgs &a, hipStream_t st) {
     if (!a.rexp || a.naux_pad % 64 != 0 || a.naux_pad > kI8MaxNauxPad || a.nvir_pad % 16 != 0 ||
-        a.i_fixed >= 0 || !a.tfull_hi || !a.tfull_lo || a.ld_full != (int64_t)a.nocc * a.nvir_pad)
+        a.i_fixed >= 0 || !a.s_hi || !a.s_lo || a.vpg < a.nvir_pad || a.g0 < 0 ||
+        a.g1 <= a.g0 || a.g1 > a.nocc || a.ld_s != (int64_t)a.nocc * a.vpg ||
+        a.ld_st != (int64_t)(a.g1 - a.g0) * a.vpg || (a.g1 < a.nocc && (!a.st_hi || !a.st_lo)) ||
+        a.nitems % a.ntp != 0)
         return hipErrorInvalidValue;
     if (a.tile == 64)
         hipLaunchKernelGGL((energy_i8x4_kernel<64, true>), dim3((unsigned)a.nitems), dim3(512), 0,

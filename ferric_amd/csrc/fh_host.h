@@ -36,8 +36,8 @@ enum SlotId {
     kSlotDressX,     // dressing: (Q|nu i) half-transformed rows
     kSlotDressY,     // dressing: undressed MO rows (Q|ia)
     kSlotB2Split,    // INT8X4 adjoint: BF16 hi and lo of the Q-major copy of B
-    kSlotTSplit,     // INT8X4 adjoint: BF16 hi and lo of the whole symmetric T~
-    kSlotG32,        // INT8X4 adjoint: FP32 X = B T~ of the rocBLAS GEMMs
+    kSlotTSplit,     // INT8X4 adjoint: BF16 hi and lo of a row group's T~ slabs S and St
+    kSlotG32,        // INT8X4 adjoint: FP32 gamma over the padded index space
     kSlotPairs,      // pair energies: (e_os, e_ss) per unique pair
     kSlotPnoWork,    // PNO: five nvir x nvir matrices and three vectors per batched pair
     kSlotPnoMeta,    // PNO: pair list, per-pair results, masks, GEMM pointer arrays

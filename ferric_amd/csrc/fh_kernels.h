@@ -142,11 +142,16 @@ struct EnergyArgs {
     bool t_f32 = false;       // T stored FP32 (reduced-precision adjoint) or FP64
     double *item_vec = nullptr;  // per item: row sums, column sums, total of t*T~
     int vec_stride = 0;
-    // Whole-T~ adjoint (INT8X4): T~ as BF16 hi / lo over the padded (i, a) index
-    // space, T~[i*nvir_pad + a][j*nvir_pad + b] = T~_ij^ab for every i, j (the
-    // matrix is symmetric), row stride ld_full.
-    uint16_t *tfull_hi = nullptr, *tfull_lo = nullptr;
-    int64_t ld_full = 0;
+    // Row-group adjoint (INT8X4): phase A of the occupied rows [g0, g1) writes
+    // T~ (symmetric over the padded (i, a) index space, column pad vpg) as BF16
+    // hi / lo into two slabs:
+    //   S [(i - g0) vpg + a][j vpg + b]   = T~_ij^ab, i in the group, every j >= g0
+    //   St[j vpg + b][(i - g0) vpg + a]   = T~_ij^ab, i in the group, j >= g1
+    // (row strides ld_s = nocc vpg and ld_st = (g1 - g0) vpg).
+    uint16_t *s_hi = nullptr, *s_lo = nullptr, *st_hi = nullptr, *st_lo = nullptr;
+    int64_t ld_s = 0, ld_st = 0;
+    int g0 = 0, g1 = 0, vpg = 0;
+    int64_t item0 = 0;        // first item of the launch (row groups)
     // Signed factor (K = B+^T B+ - B-^T B-): packed Q columns from q_neg on
     // hold B- and enter with -1.
     int q_neg = 1 << 30;
@@ -188,9 +193,24 @@ hipError_t launch_adjoint_gemm(const GemmArgs &g, hipStream_t st);
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_energy_i8x4(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_energy_i8x4_rect(const EnergyArgs &a, hipStream_t st);
-// Phase A of the adjoint on the INT8 digit-slice kernel, every item in one
-// launch (i_fixed < 0), T~ into the whole-T~ matrix.
+// Phase A of the adjoint on the INT8 digit-slice kernel over the items of the
+// occupied rows [g0, g1) (i_fixed < 0), T~ into the group's slabs.
 hipError_t launch_adjoint_i8x4(const EnergyArgs &a, hipStream_t st);
+
+// C[m][n] += alpha sum_k A[m][k] B[n][k] for m < M, n < N, with A and B given
+// as BF16 hi / lo pairs (k contiguous, K a multiple of 32) and C FP32: the
+// three products hi*hi + hi*lo + lo*hi, FP32 accumulation, 256 x 256 tiles.
+struct Bf16x3Gemm {
+    const uint16_t *ahi, *alo;
+    int64_t lda;
+    const uint16_t *bhi, *blo;
+    int64_t ldb;
+    float *c;
+    int64_t ldc;
+    int M, N, K;
+    float alpha;
+};
+hipError_t launch_bf16x3_nt(const Bf16x3Gemm &g, hipStream_t st);
 // Opposite-spin rectangle (F64 / F32) and its row reduction + fold.
 hipError_t launch_energy_rect(const EnergyArgs &a, hipStream_t st);
 hipError_t launch_rect_reduce(const EnergyArgs &a, hipStream_t st);

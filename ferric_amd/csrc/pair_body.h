@@ -130,15 +130,32 @@ __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item 
     block_store(os, ss, w, a.item_partials + 4 * it.id);
 }
 
-// One element of the whole-T~ matrix as BF16 hi = RN(x), lo = RN(x - hi) of
-// x = RN_f32(v) (the split of pack.hip).
-__device__ __forceinline__ void tfull_store(const EnergyArgs &a, int64_t r, int64_t c, double v) {
-    const float x = (float)v;
-    const __bf16 h = (__bf16)x;
-    const __bf16 l = (__bf16)(x - (float)h);
-    const int64_t k = r * a.ld_full + c;
-    a.tfull_hi[k] = __builtin_bit_cast(uint16_t, h);
-    a.tfull_lo[k] = __builtin_bit_cast(uint16_t, l);
+// T~_ij^xy (i in the row group [g0, g1), j >= i) as BF16 hi = RN(x),
+// lo = RN(x - hi) of x = RN_f32(v) (the split of pack.hip) into the group's
+// slabs: S at row (i - g0, x), column (j, y); its mirror T~_ji^yx at S row
+// (j - g0, y), column (i, x) when j is in the group (j != i: the diagonal
+// pair's elements each have their own writer), else at St row (j, y),
+// column (i - g0, x).
+__device__ __forceinline__ void slab_store(const EnergyArgs &a, int i, int x, int j, int y,
+                                           double v) {
+    const float f = (float)v;
+    const __bf16 h = (__bf16)f;
+    const __bf16 l = (__bf16)(f - (float)h);
+    const uint16_t hb = __builtin_bit_cast(uint16_t, h), lb = __builtin_bit_cast(uint16_t, l);
+    const int64_t vp = a.vpg;
+    const int64_t k = ((int64_t)(i - a.g0) * vp + x) * a.ld_s + (int64_t)j * vp + y;
+    a.s_hi[k] = hb;
+    a.s_lo[k] = lb;
+    if (j == i) return;
+    if (j < a.g1) {
+        const int64_t m = ((int64_t)(j - a.g0) * vp + y) * a.ld_s + (int64_t)i * vp + x;
+        a.s_hi[m] = hb;
+        a.s_lo[m] = lb;
+    } else {
+        const int64_t m = ((int64_t)j * vp + y) * a.ld_st + (int64_t)(i - a.g0) * vp + x;
+        a.st_hi[m] = hb;
+        a.st_lo[m] = lb;
+    }
 }
 
 __device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double v) {
@@ -154,8 +171,8 @@ __device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double 
 // written as zeros: the phase-B/C GEMMs contract over them. The 2x2 waves
 // holding the tile pass active = true; further waves of a larger workgroup
 // (the INT8 kernel's) pass false and only join the barriers and the block sum.
-// FULLT: T~ goes to the whole-T~ matrix (a.tfull_*) at (i a, j b), (i b, j a)
-// and their mirrors instead of the row blocks tbuf / ubuf.
+// FULLT: T~ goes to the row group's slabs (slab_store) at (i a, j b) and
+// (i b, j a), with their mirrors, instead of the row blocks tbuf / ubuf.
 template <int TILE, typename Acc, typename RowOfT, typename Corr, bool FULLT = false>
 __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item &it,
                                                  const Lane &L,
@@ -210,13 +227,8 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
                     csum[y] += val;
                 }
                 if constexpr (FULLT) {
-                    const int64_t ri = (int64_t)it.i * a.nvir_pad, rj = (int64_t)it.j * a.nvir_pad;
-                    tfull_store(a, ri + aa, rj + bb, T1);
-                    tfull_store(a, rj + bb, ri + aa, T1);
-                    if (!diag_tile) {
-                        tfull_store(a, ri + bb, rj + aa, T2);
-                        tfull_store(a, rj + aa, ri + bb, T2);
-                    }
+                    slab_store(a, it.i, aa, it.j, bb, T1);
+                    if (!diag_tile) slab_store(a, it.i, bb, it.j, aa, T2);
                 } else {
                     // T~_ij^ab and T~_ij^ba into the row-i blocks.
                     tstore(a.tbuf, a.t_f32,

@@ -58,12 +58,28 @@ __global__ void k_i8(const int *a, const int *b, int *c) {
     for (int v = 0; v < 4; ++v) c[4 * l + v] = acc[v];
 }
 
+typedef float v16f __attribute__((ext_vector_type(16)));
+__global__ void k_bf16_32(const uint16_t *a, const uint16_t *b, float *c) {
+    const int l = threadIdx.x;
+    v8bf x, y;
+    for (int e = 0; e < 8; ++e) {
+        x[e] = __builtin_bit_cast(__bf16, a[8 * l + e]);
+        y[e] = __builtin_bit_cast(__bf16, b[8 * l + e]);
+    }
+    v16f acc;
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc, 0, 0, 0);
+    for (int v = 0; v < 16; ++v) c[16 * l + v] = acc[v];
+}
+
 // kind 0 = f64 16x16x4, 1 = f32 16x16x4, 2 = bf16 16x16x32, 3 = scaled f8
-// 16x16x128, 4 = i8 16x16x64 (16 bytes per lane, int32 result).
+// 16x16x128, 4 = i8 16x16x64 (16 bytes per lane, int32 result), 5 = bf16
+// 32x32x16 (16 bytes per lane, 16 floats per lane).
 extern "C" int probe_mfma(int kind, const void *a, const void *b, const int *sa, const int *sb,
                           void *c) {
     size_t ab = kind == 0 ? 64 * 8 : kind == 1 ? 64 * 4 : kind == 2 || kind == 4 ? 64 * 16 : 64 * 32;
-    size_t cb = kind == 0 ? 64 * 4 * 8 : 64 * 4 * 4;
+    if (kind == 5) ab = 64 * 16;
+    size_t cb = kind == 0 ? 64 * 4 * 8 : kind == 5 ? 64 * 16 * 4 : 64 * 4 * 4;
     void *da, *db, *dc;
     int *dsa = nullptr, *dsb = nullptr;
     if (hipMalloc(&da, ab) || hipMalloc(&db, ab) || hipMalloc(&dc, cb)) return 1;
@@ -81,6 +97,7 @@ extern "C" int probe_mfma(int kind, const void *a, const void *b, const int *sa,
     case 2: hipLaunchKernelGGL(k_bf16, dim3(1), dim3(64), 0, 0, (const uint16_t *)da, (const uint16_t *)db, (float *)dc); break;
     case 3: hipLaunchKernelGGL(k_f8, dim3(1), dim3(64), 0, 0, (const int *)da, (const int *)db, (const int *)dsa, (const int *)dsb, (float *)dc); break;
     case 4: hipLaunchKernelGGL(k_i8, dim3(1), dim3(64), 0, 0, (const int *)da, (const int *)db, (int *)dc); break;
+    case 5: hipLaunchKernelGGL(k_bf16_32, dim3(1), dim3(64), 0, 0, (const uint16_t *)da, (const uint16_t *)db, (float *)dc); break;
     default: return 4;
     }
     if (hipGetLastError() != hipSuccess) return 5;

@@ -106,6 +106,25 @@ def test_bf16_16x16x32(probe):
     assert np.allclose(_to_layout(c), want, rtol=1e-5, atol=1e-5)
 
 
+def test_bf16_32x32x16(probe):
+    """v_mfma_f32_32x32x16_bf16 (the adjoint's Gamma GEMM): lane l = 32 h + c
+    supplies A[row c][k = 8h .. 8h + 7] and B[k = 8h ..][col c]; result element
+    j of lane l is C[8 (j / 4) + 4 h + j % 4][c]."""
+    rng = np.random.default_rng(5)
+    a = _bf16_bits(rng.standard_normal((64, 8)))
+    b = _bf16_bits(rng.standard_normal((64, 8)))
+    c = _run(probe, 5, a, b, np.zeros((64, 16), np.float32))
+    av = _bf16_val(a).reshape(2, 32, 8)
+    bv = _bf16_val(b).reshape(2, 32, 8)
+    want = np.einsum("hre,hce->rc", av, bv)
+    got = np.zeros((32, 32))
+    for lane in range(64):
+        h, col = divmod(lane, 32)
+        for j in range(16):
+            got[8 * (j // 4) + 4 * h + j % 4, col] = c[lane, j]
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-5)
+
+
 def e4m3_decode(codes):
     c = codes.astype(np.int64)
     s = np.where(c & 0x80, -1.0, 1.0)
