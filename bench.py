@@ -118,8 +118,9 @@ def mfma_sustained(prec_name):
     return rate
 
 
-def cpu_baseline(p, rows, nthreads):
-    """The oracle's OpenBLAS restatement of the reference CPU path on a sample of rows."""
+def cpu_baseline(p, rows, nthreads, reps):
+    """The oracle's OpenBLAS restatement of the reference CPU path on a sample of
+    rows, timed reps times: (FLOP, [seconds per rep])."""
     import numpy as np
 
     import oracle
@@ -127,10 +128,12 @@ def cpu_baseline(p, rows, nthreads):
     oracle.build()
     b64 = np.ascontiguousarray(p.b_ov, dtype=np.float64)
     flops = sum(2.0 * p.nvir ** 2 * p.naux * (p.nocc - i) for i in rows)
-    t0 = time.perf_counter()
-    res = oracle.spin_components_blas(b64, p.eps, p.nocc, p.nvir, nthreads=nthreads, i_rows=rows)
-    dt = time.perf_counter() - t0
-    return flops, dt, res
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.spin_components_blas(b64, p.eps, p.nocc, p.nvir, nthreads=nthreads, i_rows=rows)
+        ts.append(time.perf_counter() - t0)
+    return flops, ts
 
 
 def main():
@@ -149,6 +152,7 @@ def main():
     ap.add_argument("--no-small-configs", action="store_true",
                     help="skip the water-shape and decane-scale legs")
     ap.add_argument("--cpu-rows", default="50:100", help="occupied rows of the CPU sample")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed repetitions of the CPU sample")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (calibrates HBM counters)")
     ap.add_argument("--probe-rates", action="store_true",
@@ -407,7 +411,7 @@ def main():
         dc = synthetic.decane_scale()
         db = torch.from_numpy(dc.b_ov).to(dev)
         row = {}
-        for prec in (Precision.FP64, Precision.FP32):
+        for prec in (Precision.FP64, Precision.FP32, Precision.MIXED):
             sc = rimp2.spin_components(db, dc.eps, dc.nocc, dc.nvir, precision=prec)
             med, mn = timed_ms(lambda: rimp2.spin_components(db, dc.eps, dc.nocc, dc.nvir,
                                                              precision=prec), 10)
@@ -415,13 +419,14 @@ def main():
                               "ms_median": med, "ms_min": mn,
                               "tflops": dc.flops / (med * 1e-3) / 1e12}
         row["FP32"]["dE_vs_fp64"] = row["FP32"]["E_total"] - row["FP64"]["E_total"]
+        row["MIXED"]["dE_vs_fp64"] = row["MIXED"]["E_total"] - row["FP64"]["E_total"]
         small[dc.name] = row
         log(f"{dc.name}: FP64 E {row['FP64']['E_total']:.12f} ({row['FP64']['ms_median']:.2f} ms), "
             f"FP32 dE {row['FP32']['dE_vs_fp64']:+.2e} ({row['FP32']['ms_median']:.2f} ms)")
 
         adj_small = {}
         outs = {}
-        for prec in (Precision.FP64, Precision.BF16X3):
+        for prec in (Precision.FP64, Precision.BF16X3, Precision.INT8X4):
             outs[prec.name] = energy_adjoint(db, dc.eps, dc.nocc, dc.nvir, 0, dc.nocc, 1.2,
                                              1.0 / 3.0, prec)
             med, mn = timed_ms(lambda: energy_adjoint(db, dc.eps, dc.nocc, dc.nvir, 0, dc.nocc,
@@ -429,15 +434,27 @@ def main():
             adj_small[prec.name] = {"E_scs": outs[prec.name][1], "ms_median": med, "ms_min": mn,
                                     "tflops_3FE": 3 * dc.flops / (med * 1e-3) / 1e12}
         g64, d64 = outs["FP64"][2], np.asarray(outs["FP64"][3])
-        g16, d16 = outs["BF16X3"][2], np.asarray(outs["BF16X3"][3])
-        adj_small["BF16X3"]["dE_vs_fp64"] = adj_small["BF16X3"]["E_scs"] - adj_small["FP64"]["E_scs"]
-        adj_small["BF16X3"]["max_rel_dGamma"] = float((g16 - g64).abs().max() / g64.abs().max())
-        adj_small["BF16X3"]["max_rel_dEps"] = float(np.max(np.abs(d16 - d64)) / np.max(np.abs(d64)))
+        for name in ("BF16X3", "INT8X4"):
+            gq, dq = outs[name][2], np.asarray(outs[name][3])
+            adj_small[name]["dE_vs_fp64"] = adj_small[name]["E_scs"] - adj_small["FP64"]["E_scs"]
+            adj_small[name]["max_rel_dGamma"] = float((gq - g64).abs().max() / g64.abs().max())
+            adj_small[name]["max_rel_dEps"] = float(np.max(np.abs(dq - d64)) / np.max(np.abs(d64)))
         small[dc.name + " adjoint (SCS 1.2, 1/3)"] = adj_small
+        # MP2 densities P_oo, P_vv (FP64; the amplitudes of every ordered pair held)
+        sc_d, p_oo, p_vv = rimp2.mp2_density(db, dc.eps, dc.nocc, dc.nvir, 0, dc.nocc, 1.2,
+                                             1.0 / 3.0)
+        med, mn = timed_ms(lambda: rimp2.mp2_density(db, dc.eps, dc.nocc, dc.nvir, 0, dc.nocc,
+                                                     1.2, 1.0 / 3.0), 5)
+        dd = 2 * np.concatenate([np.diag(p_oo), np.diag(p_vv)])
+        small[dc.name + " MP2 densities (SCS 1.2, 1/3)"] = {
+            "ms_median": med, "ms_min": mn,
+            "max_rel_diag_vs_adjoint_deps": float(np.max(np.abs(dd - d64)) / np.max(np.abs(d64))),
+            "trace_sum": float(np.trace(p_oo) + np.trace(p_vv))}
+        log(f"{dc.name} MP2 densities: {med:.2f} ms")
         log(f"{dc.name} adjoint: FP64 {adj_small['FP64']['ms_median']:.1f} ms, BF16X3 "
             f"{adj_small['BF16X3']['ms_median']:.1f} ms, rel dGamma "
             f"{adj_small['BF16X3']['max_rel_dGamma']:.1e}")
-        del db, wb, outs, g64, g16
+        del db, wb, outs, g64
 
         # Open shell at decane scale (alpha 21 occ / 229 vir, beta 20 / 230, naux
         # 960, same distributions): same-spin blocks through the pair kernel and
@@ -560,17 +577,41 @@ def main():
     if not args.no_cpu_baseline:
         lo, hi = (int(x) for x in args.cpu_rows.split(":"))
         rows = list(range(max(lo, 0), min(hi, p.nocc)))
-        nthreads = min(16, len(os.sched_getaffinity(0)))
-        flops, dt, cres = cpu_baseline(p, rows, nthreads)
-        rate = flops / dt / 1e12
+        nthreads = len(os.sched_getaffinity(0))   # every host core this process may use
+        flops, ts = cpu_baseline(p, rows, nthreads, args.cpu_reps)
+        dt_min, dt_med = float(np.min(ts)), float(np.median(ts))
+        rate = flops / dt_min / 1e12
         cpu = {
             "value": rate, "unit": "TFLOP/s", "cores": nthreads, "kind": "port",
             "sample": (f"occupied rows {rows[0]}..{rows[-1]} of {p.name} "
-                       f"({flops:.3e} of {F:.3e} FLOP, {dt:.1f} s wall, OpenBLAS tail GEMMs, "
-                       f"one task per row, {cpu_model()})"),
+                       f"({flops:.3e} of {F:.3e} FLOP; min {dt_min:.2f} s, median {dt_med:.2f} s "
+                       f"over {len(ts)} reps; OpenBLAS tail GEMMs, one task per row, "
+                       f"{nthreads} threads, {cpu_model()})"),
+            "value_median": flops / dt_med / 1e12,
             "time_to_ecorr_s_extrapolated": F / (rate * 1e12),
         }
-        log(f"cpu baseline: {rate:.3f} TFLOP/s on {nthreads} threads ({dt:.1f} s)")
+        log(f"cpu baseline: {rate:.3f} TFLOP/s (min of {len(ts)}) on {nthreads} threads "
+            f"({dt_min:.1f} s min, {dt_med:.1f} s median)")
+        # Parity of the sample: the GPU FP64 per-row partials against the
+        # oracle's partials of the same rows, with each row's element bound.
+        import oracle
+
+        _, gpu_rows = rimp2.spin_components(b, p.eps, p.nocc, p.nvir, precision=Precision.FP64,
+                                            row_partials=True)
+        chk = rows[::max(1, len(rows) // 4)]
+        want = oracle.row_partials_blas(np.ascontiguousarray(p.b_ov, dtype=np.float64), p.eps,
+                                        p.nocc, p.nvir, chk)
+        worst = 0.0
+        for i, (eos, ess, bos, bss) in want.items():
+            worst = max(worst, abs(gpu_rows[i, 0] - eos) / bos, abs(gpu_rows[i, 1] - ess) / bss)
+        cpu["gpu_fp64_rows_vs_oracle"] = {
+            "rows": [int(i) for i in want],
+            "max_abs_dE_row": float(max(max(abs(gpu_rows[i, 0] - w[0]), abs(gpu_rows[i, 1] - w[1]))
+                                        for i, w in want.items())),
+            "max_ratio_to_row_bound": worst}
+        log(f"GPU FP64 rows {list(want)} vs oracle: max |dE_row| "
+            f"{cpu['gpu_fp64_rows_vs_oracle']['max_abs_dE_row']:.2e} Eh, "
+            f"{worst:.2e} of the element bound")
         if small:
             # The whole decane-scale energy on the host, min and median of 5, and its
             # difference from the GPU FP64 result.

@@ -328,8 +328,16 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
             for (int y = 0; y < NB; ++y) g2[x][y] = g1[x][y];
     }
     if constexpr (ADJ) {
+        // xbuf is free again: every read of it above precedes this point for
+        // all waves (the single-product merge ends in a barrier; the C2
+        // handover reads gx before the barrier inside the epilogue, after
+        // which tlds is read).
+        static_assert(sizeof(xbuf) >= 2 * TILE * (TILE + 1) * sizeof(uint32_t),
+                      "xbuf holds the two transposed T~ tiles");
+        __syncthreads();
         pair::adjoint_epilogue<TILE, v4d, pair::RowOf<float>, pair::NoCorr, true>(
-            a, it, L, g1, g2, rv, cv, pair::RowOf<float>{}, pair::NoCorr{}, prod == 0);
+            a, it, L, g1, g2, rv, cv, pair::RowOf<float>{}, pair::NoCorr{}, prod == 0,
+            (uint32_t *)xbuf);
         return;
     }
     // The energy terms (pair::energy_epilogue's, with the virtual energies

@@ -130,31 +130,40 @@ __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item 
     block_store(os, ss, w, a.item_partials + 4 * it.id);
 }
 
-// T~_ij^xy (i in the row group [g0, g1), j >= i) as BF16 hi = RN(x),
-// lo = RN(x - hi) of x = RN_f32(v) (the split of pack.hip) into the group's
-// slabs: S at row (i - g0, x), column (j, y); its mirror T~_ji^yx at S row
-// (j - g0, y), column (i, x) when j is in the group (j != i: the diagonal
-// pair's elements each have their own writer), else at St row (j, y),
-// column (i - g0, x).
-__device__ __forceinline__ void slab_store(const EnergyArgs &a, int i, int x, int j, int y,
-                                           double v) {
+// BF16 hi = RN(x), lo = RN(x - hi) of x = RN_f32(v) (the split of pack.hip),
+// packed as hi | lo << 16.
+__device__ __forceinline__ uint32_t bf16_pair(double v) {
     const float f = (float)v;
     const __bf16 h = (__bf16)f;
     const __bf16 l = (__bf16)(f - (float)h);
-    const uint16_t hb = __builtin_bit_cast(uint16_t, h), lb = __builtin_bit_cast(uint16_t, l);
+    return (uint32_t)__builtin_bit_cast(uint16_t, h) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16);
+}
+
+// The row group's T~ slabs (EnergyArgs::s_hi, st_hi): slab_put writes T~_ij^xy
+// (i in [g0, g1), j >= i) at S row (i - g0, x), column (j, y); slab_put_mirror
+// writes its mirror T~_ji^yx at S row (j - g0, y), column (i, x) when j is in
+// the group, else at St row (j, y), column (i - g0, x) (nothing for j == i:
+// the diagonal pair's elements each have their own writer).
+__device__ __forceinline__ void slab_put(const EnergyArgs &a, int i, int x, int j, int y,
+                                         uint32_t hl) {
     const int64_t vp = a.vpg;
     const int64_t k = ((int64_t)(i - a.g0) * vp + x) * a.ld_s + (int64_t)j * vp + y;
-    a.s_hi[k] = hb;
-    a.s_lo[k] = lb;
+    a.s_hi[k] = (uint16_t)hl;
+    a.s_lo[k] = (uint16_t)(hl >> 16);
+}
+__device__ __forceinline__ void slab_put_mirror(const EnergyArgs &a, int i, int x, int j, int y,
+                                                uint32_t hl) {
     if (j == i) return;
+    const int64_t vp = a.vpg;
     if (j < a.g1) {
         const int64_t m = ((int64_t)(j - a.g0) * vp + y) * a.ld_s + (int64_t)i * vp + x;
-        a.s_hi[m] = hb;
-        a.s_lo[m] = lb;
+        a.s_hi[m] = (uint16_t)hl;
+        a.s_lo[m] = (uint16_t)(hl >> 16);
     } else {
         const int64_t m = ((int64_t)j * vp + y) * a.ld_st + (int64_t)(i - a.g0) * vp + x;
-        a.st_hi[m] = hb;
-        a.st_lo[m] = lb;
+        a.st_hi[m] = (uint16_t)hl;
+        a.st_lo[m] = (uint16_t)(hl >> 16);
     }
 }
 
@@ -171,15 +180,20 @@ __device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double 
 // written as zeros: the phase-B/C GEMMs contract over them. The 2x2 waves
 // holding the tile pass active = true; further waves of a larger workgroup
 // (the INT8 kernel's) pass false and only join the barriers and the block sum.
-// FULLT: T~ goes to the row group's slabs (slab_store) at (i a, j b) and
-// (i b, j a), with their mirrors, instead of the row blocks tbuf / ubuf.
+// FULLT: T~ goes to the row group's slabs at (i a, j b) and (i b, j a), with
+// their mirrors, instead of the row blocks tbuf / ubuf. A lane's elements run
+// along b, so the two of the four writes whose slab rows are indexed by b
+// (T~_ij^ab's mirror and T~_ij^ba) first go through LDS (tlds, two TILE x
+// (TILE + 1) words) and are written after a barrier by all threads of the
+// workgroup running along a: every slab store is then a contiguous run.
 template <int TILE, typename Acc, typename RowOfT, typename Corr, bool FULLT = false>
 __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item &it,
                                                  const Lane &L,
                                                  Acc (&acc1)[TILE / 32][TILE / 32],
                                                  Acc (&acc2)[TILE / 32][TILE / 32],
                                                  const bool *rv, const bool *cv, RowOfT row_of,
-                                                 Corr corr, bool active = true) {
+                                                 Corr corr, bool active = true,
+                                                 uint32_t *tlds = nullptr) {
     constexpr int TW = TILE / 2, NB = TW / 16;
     const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
     const double cd = a.c_os + a.c_ss, ce = a.c_ss;
@@ -227,8 +241,17 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
                     csum[y] += val;
                 }
                 if constexpr (FULLT) {
-                    slab_store(a, it.i, aa, it.j, bb, T1);
-                    if (!diag_tile) slab_store(a, it.i, bb, it.j, aa, T2);
+                    constexpr int LT = TILE + 1;
+                    const uint32_t h1 = bf16_pair(T1);
+                    const int la = L.wr * TW + x * 16 + row_of(L.g, v);
+                    const int lb = L.wc * TW + y * 16 + L.r;
+                    slab_put(a, it.i, aa, it.j, bb, h1);
+                    tlds[la * LT + lb] = h1;
+                    if (!diag_tile) {
+                        const uint32_t h2 = bf16_pair(T2);
+                        slab_put_mirror(a, it.i, bb, it.j, aa, h2);
+                        tlds[TILE * LT + la * LT + lb] = h2;
+                    }
                 } else {
                     // T~_ij^ab and T~_ij^ba into the row-i blocks.
                     tstore(a.tbuf, a.t_f32,
@@ -284,6 +307,18 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
         double s = 0.0;
         for (int k = 0; k < rmax; ++k) s += rows_lds[0][k] + rows_lds[1][k];
         vec[2 * TILE] = s;
+    }
+    if constexpr (FULLT) {
+        // The transposed half of the slab writes (tlds was filled before the
+        // barrier above): thread runs along a, the slab rows are indexed by b.
+        constexpr int LT = TILE + 1;
+        for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
+            const int la = e % TILE, lb = e / TILE;
+            const int aa = L.A0 + la, bb = L.B0 + lb;
+            if (aa >= a.nvir_pad || bb >= a.nvir_pad) continue;
+            slab_put_mirror(a, it.i, aa, it.j, bb, tlds[la * LT + lb]);
+            if (!diag_tile) slab_put(a, it.i, bb, it.j, aa, tlds[TILE * LT + la * LT + lb]);
+        }
     }
 }
 
