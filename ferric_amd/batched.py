@@ -18,17 +18,25 @@ from . import _lib
 from ._lib import LayoutError, Precision, check, lib
 
 
+_DOMAIN_DTYPE = np.dtype([("b_offset", "<u8"), ("eps_offset", "<u8"), ("nocc", "<u4"),
+                          ("nvir", "<u4"), ("naux", "<u4"), ("reserved", "<u4")])
+
+
 def descriptors(nocc, nvir, naux, b_offset, eps_offset) -> "ctypes.Array":
+    """The fh_domain array (built as one numpy record array, then viewed as ctypes)."""
     n = len(nocc)
-    arr = (_lib.DomainC * n)()
-    for k in range(n):
-        arr[k].b_offset = int(b_offset[k])
-        arr[k].eps_offset = int(eps_offset[k])
-        arr[k].nocc = int(nocc[k])
-        arr[k].nvir = int(nvir[k])
-        arr[k].naux = int(naux[k])
-        arr[k].reserved = 0
-    return arr
+    if _DOMAIN_DTYPE.itemsize != ctypes.sizeof(_lib.DomainC):
+        raise LayoutError("fh_domain layout differs from the binding's record")
+    rec = np.zeros(n, dtype=_DOMAIN_DTYPE)
+    for name, v in (("b_offset", b_offset), ("eps_offset", eps_offset), ("nocc", nocc),
+                    ("nvir", nvir), ("naux", naux)):
+        v = np.asarray(v)
+        if v.shape != (n,) or (v.size and int(v.min()) < 0):
+            raise LayoutError(f"domain field {name} must be {n} non-negative integers")
+        if name in ("nocc", "nvir", "naux") and v.size and int(v.max()) >= 1 << 32:
+            raise LayoutError(f"domain field {name} beyond 32 bits")
+        rec[name] = v
+    return (_lib.DomainC * n).from_buffer_copy(rec.tobytes())
 
 
 def domain_energies(b_flat, eps_flat, nocc, nvir, naux, b_offset, eps_offset,

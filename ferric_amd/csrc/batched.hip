@@ -1,25 +1,38 @@
 // Batched local-MP2 pair domains: many independent RI-MP2 pair-energy
-// problems of uneven shape in one persistent launch per tile class (gfx950).
+// problems of uneven shape in one persistent launch (gfx950).
 //
 // The reference evaluates its pair domains one after another on the host
 // (crates/ferric-mp2/src/dlpno_mp2.rs:718-745, one nvir x nvir GEMM and pair
 // loop per pair). Here every (domain, occupied pair i <= j, virtual tile pair)
-// is one work item of a device-wide queue; workgroups of a persistent kernel
-// pull items, most expensive domains first, and run the same fused
-// contraction + FP64 epilogue as the one-problem kernels (pair_body.h). Each
-// item writes its own compensated partial, and one workgroup per domain folds
-// its items in item order, so the energies do not depend on which workgroup
-// ran which item. The tile side of each domain comes from the tile table
-// (tuning.hip, filled by ferric_amd/autotune.py); domains are grouped by tile
-// side, one persistent launch per group. The whole sequence (queue reset,
-// packing, persistent launches, reductions) can be captured once in a hipGraph
-// and replayed for the same descriptor set.
+// is one work item; workgroups of one persistent kernel pull items from eight
+// queues, one per XCD, run the fused contraction and FP64 epilogue of the
+// one-problem kernels (pair_body.h), and write one compensated partial per
+// item; one workgroup per domain then folds its items in item order, so the
+// energies do not depend on which workgroup ran which item.
+//
+//  - Domains are dealt to the eight queues largest first (each to the queue
+//    with the least work so far); a workgroup starts on the queue of its own
+//    XCD (workgroup b runs on XCD b % 8), so a domain's operand rows are read
+//    into one L2, and takes items from the other queues once its own is empty.
+//  - Inside a queue the domains run most expensive item first.
+//  - Each domain has its own tile side (32 or 64, from the tile table of
+//    tuning.hip, filled by ferric_amd/autotune.py); the kernel branches on it
+//    per item, over one shared LDS buffer.
+//  - FP64 reads the operand panels straight from the caller's Q-major blocks:
+//    no packing pass. BF16x3 packs hi / lo rows first (the split needs a pass
+//    of its own for the sum of lo^2 per row).
+//  - Thread 0 asks for its next item as soon as it starts one, so the queue
+//    atomic's latency hides behind the item's work.
+//
+// The whole sequence (queue reset, packing, persistent launch, reductions) can
+// be captured once in a hipGraph and replayed for the same descriptor set.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -36,8 +49,8 @@ namespace {
 struct DomDev {
     int64_t b_off;      // caller B (row-major (naux, nocc*nvir)) offset, elements
     int64_t eps_off;    // [occupied..., virtual...] offset
-    int64_t bp_off;     // packed rows offset (elements), same in the lo arena
-    int64_t row_off;    // first packed row (for the BF16 diagonal correction)
+    int64_t bp_off;     // BF16: packed rows offset (elements), same in the lo arena
+    int64_t row_off;    // BF16: first packed row (for the diagonal correction)
     int64_t item0;      // first work item of the domain
     int nocc, nvir, naux, nvir_pad, naux_pad, ntile, ntp, tile;
 };
@@ -54,20 +67,22 @@ struct BatchArgs {
     const int *item_dom;      // domain of each work item
     const PackTile *ptiles;
     int64_t nptiles;
-    void *op0;                // packed FP64 rows, or BF16 hi
+    void *op0;                // BF16 hi
     void *op1;                // BF16 lo
     double *corr;             // BF16 sum_Q lo^2 per packed row
     int64_t nrows;            // packed rows over all domains
     const int64_t *row_dom_start;  // first packed row of each domain (ndom + 1)
     int ndom;
     double *item_partials;
-    int *counters;            // one queue counter per tile class
+    int *counters;            // one queue counter per XCD
+    int qbegin[kXcds + 1];    // queue x holds items [qbegin[x], qbegin[x + 1])
     double *out;              // (e_os, e_ss) per domain
 };
 
-// ---- packing -------------------------------------------------------------------
+// ---- packing (BF16x3) -------------------------------------------------------------
 // One 64 x 64 tile of one domain: rows r of the domain's packed block, Q in
-// [q0, q0 + 64). Same layout and zero padding as pack.hip.
+// [q0, q0 + 64), split into BF16 hi / lo. Same layout and zero padding as
+// pack.hip.
 __global__ __launch_bounds__(256) void batched_pack_kernel(BatchArgs a) {
     __shared__ double tile[64][65];
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
@@ -95,16 +110,11 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(BatchArgs a) {
             const int64_t orow = pt.r0 + rr;
             if (orow < nrows && pt.q0 + tx < dm.naux_pad) {
                 const int64_t o = dm.bp_off + orow * dm.naux_pad + pt.q0 + tx;
-                const double x = tile[tx][rr];
-                if (a.prec == Prec::F64) {
-                    ((double *)a.op0)[o] = x;
-                } else {
-                    const float xf = (float)x;
-                    const __bf16 h = (__bf16)xf;
-                    const __bf16 l = (__bf16)(xf - (float)h);
-                    ((uint16_t *)a.op0)[o] = __builtin_bit_cast(uint16_t, h);
-                    ((uint16_t *)a.op1)[o] = __builtin_bit_cast(uint16_t, l);
-                }
+                const float xf = (float)tile[tx][rr];
+                const __bf16 h = (__bf16)xf;
+                const __bf16 l = (__bf16)(xf - (float)h);
+                ((uint16_t *)a.op0)[o] = __builtin_bit_cast(uint16_t, h);
+                ((uint16_t *)a.op1)[o] = __builtin_bit_cast(uint16_t, l);
             }
         }
     }
@@ -139,55 +149,292 @@ __global__ void reset_counters_kernel(int *c, int n) {
     if (threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
-// ---- persistent pair kernel ------------------------------------------------------
-// Items [item_begin, item_end) of one tile class, taken from a device-wide
-// counter. Every workgroup exits once the counter passes item_end.
-template <int TILE, bool BF16>
-__global__ __launch_bounds__(256) void batched_pair_kernel(BatchArgs a, int64_t item_begin,
-                                                           int64_t item_end, int cls) {
-    __shared__ __attribute__((aligned(16))) pair::K4Lds<double, TILE, 16> lds64;
-    __shared__ __attribute__((aligned(16))) pair::Bf16Lds<TILE> ldsbf;
-    __shared__ int64_t s_item;
-    EnergyArgs e;
-    e.prec = BF16 ? Prec::BF16X3 : Prec::F64;
-    e.tile = TILE;
+// ---- work queues -------------------------------------------------------------------
+// The next item for a workgroup whose current queue is q (0..7): queue q's
+// next item while it lasts, then the queues after it in turn. Returns -1 when
+// all eight are empty. Called by one thread.
+__device__ __forceinline__ int next_item(const BatchArgs &a, int &q, int &tried) {
+    while (tried < kXcds) {
+        const int it = a.qbegin[q] + atomicAdd(&a.counters[q], 1);
+        if (it < a.qbegin[q + 1]) return it;
+        q = (q + 1) % kXcds;
+        ++tried;
+    }
+    return -1;
+}
+
+// The descriptor of the workgroup's current domain, held in scalar registers
+// (the item index is uniform, so every lane loaded the same values; the
+// compiler cannot see that through a vector load).
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ DomDev load_domain(const BatchArgs &a, int item) {
+    const DomDev d = a.dom[__builtin_amdgcn_readfirstlane(a.item_dom[item])];
+    DomDev u;
+    u.b_off = uniform64(d.b_off);
+    u.eps_off = uniform64(d.eps_off);
+    u.bp_off = uniform64(d.bp_off);
+    u.row_off = uniform64(d.row_off);
+    u.item0 = uniform64(d.item0);
+    u.nocc = __builtin_amdgcn_readfirstlane(d.nocc);
+    u.nvir = __builtin_amdgcn_readfirstlane(d.nvir);
+    u.naux = __builtin_amdgcn_readfirstlane(d.naux);
+    u.nvir_pad = __builtin_amdgcn_readfirstlane(d.nvir_pad);
+    u.naux_pad = __builtin_amdgcn_readfirstlane(d.naux_pad);
+    u.ntile = __builtin_amdgcn_readfirstlane(d.ntile);
+    u.ntp = __builtin_amdgcn_readfirstlane(d.ntp);
+    u.tile = __builtin_amdgcn_readfirstlane(d.tile);
+    return u;
+}
+
+// The item's coordinates and the EnergyArgs fields of its domain.
+__device__ __forceinline__ void item_setup(const BatchArgs &a, const DomDev &dm, int item,
+                                           pair::Item &it, EnergyArgs &e) {
+    const int64_t local = item - dm.item0;
+    it.id = item;
+    decode_pair(local / dm.ntp, dm.nocc, it.i, it.j);
+    decode_tile_pair((int)(local % dm.ntp), dm.ntile, it.ta, it.tb);
+    e.nocc = dm.nocc;
+    e.nvir = dm.nvir;
+    e.nvir_pad = dm.nvir_pad;
+    e.naux_pad = dm.naux_pad;
+    e.ntile = dm.ntile;
+    e.ntp = dm.ntp;
+    e.eps_occ = a.eps + dm.eps_off;
+    e.eps_vir = a.eps + dm.eps_off + dm.nocc;
+}
+
+__device__ __forceinline__ void energy_args_init(const BatchArgs &a, Prec prec, EnergyArgs &e) {
+    e.prec = prec;
+    e.tile = 0;
+    e.op0 = e.op1 = nullptr;
     e.scale = nullptr;
+    e.diag_corr = nullptr;
     e.nqb = 0;
     e.kappa = NAN;
     e.item_partials = a.item_partials;
     e.row_partials = nullptr;
     e.result = nullptr;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = item_begin + atomicAdd(&a.counters[cls], 1);
-        __syncthreads();
-        const int64_t item = s_item;
-        __syncthreads();
-        if (item >= item_end) break;
-        const DomDev dm = a.dom[a.item_dom[item]];
-        const int64_t local = item - dm.item0;
-        pair::Item it;
-        it.id = item;
-        decode_pair(local / dm.ntp, dm.nocc, it.i, it.j);
-        decode_tile_pair((int)(local % dm.ntp), dm.ntile, it.ta, it.tb);
-        e.nocc = dm.nocc;
-        e.nvir = dm.nvir;
-        e.nvir_pad = dm.nvir_pad;
-        e.naux_pad = dm.naux_pad;
-        e.ntile = dm.ntile;
-        e.ntp = dm.ntp;
-        e.eps_occ = a.eps + dm.eps_off;
-        e.eps_vir = a.eps + dm.eps_off + dm.nocc;
-        if constexpr (BF16) {
-            e.op0 = (const uint16_t *)a.op0 + dm.bp_off;
-            e.op1 = (const uint16_t *)a.op1 + dm.bp_off;
-            e.diag_corr = a.corr + dm.row_off;
-            pair::pair_bf16x3_body<TILE, false>(e, it, ldsbf);
-        } else {
-            e.op0 = (const double *)a.op0 + dm.bp_off;
-            e.op1 = nullptr;
-            e.diag_corr = nullptr;
-            pair::pair_k4_body<double, TILE, 16, false>(e, it, lds64);
+}
+
+// ---- FP64 items, operand panels straight from the caller's blocks -----------------
+// A panel is KP consecutive Q rows of the item's four operand row ranges
+// (0 = (i, A), 1 = (j, B), 2 = (j, A), 3 = (i, B); 2 and 3 are 0 and 1 one
+// panel further on for a single-product item), read from the domain's
+// row-major B[q][i nvir + a] with a fastest across the threads (TILE * 8
+// contiguous bytes per Q row) and kept in LDS as [operand][q][a]. Values past
+// nvir or naux are zeros. Fragment reads: lane (r, g) takes A[row r][k] and
+// B[k][col r] for the KPL Q rows k = g KPL + s of the panel, so a half-wave
+// reads 16 consecutive doubles in two Q rows KPL apart; KPL * LDA * 8 = 1152
+// = 128 (mod 256) bytes puts them on disjoint halves of the banks.
+// Dword 3 of a raw (untyped, stride 0) gfx9 buffer resource.
+constexpr int kRawBufferFlags = 0x00020000;
+
+template <int TILE>
+struct QPanel;
+template <>
+struct QPanel<64> {
+    static constexpr int KP = 8, LDA = 72;
+};
+template <>
+struct QPanel<32> {
+    static constexpr int KP = 16, LDA = 36;
+};
+template <int TILE>
+using QLds = double[2][4][QPanel<TILE>::KP][QPanel<TILE>::LDA];
+union BatchLds64 {
+    QLds<64> t64;
+    QLds<32> t32;
+};
+static_assert(sizeof(QLds<64>) == sizeof(QLds<32>), "both tile sides fill the same LDS");
+
+template <int TILE>
+__device__ __forceinline__ void fp64_item(const BatchArgs &ba, const DomDev &dm,
+                                          const pair::Item &it, const EnergyArgs &e,
+                                          QLds<TILE> &lds) {
+    using namespace pair;
+    constexpr int KP = QPanel<TILE>::KP, KPL = KP / 4;
+    constexpr int TW = TILE / 2, NB = TW / 16;
+    constexpr int RPT = 256 / TILE;            // Q rows per pass of the workgroup
+    constexpr int EPT = 8;                     // elements per thread per panel
+    static_assert(KP == 2 * RPT, "two passes per operand");
+    using M = Mfma4<double>;
+
+    const int tid = threadIdx.x;
+    const int nvir = dm.nvir, naux = dm.naux;
+    const int64_t nov = (int64_t)dm.nocc * nvir;
+    const int A0 = it.ta * TILE, B0 = it.tb * TILE;
+    const SplitQ sq(it, dm.naux_pad, KP);
+    // Thread tid loads column a = tid % TILE of every operand, in Q rows
+    // qr + RPT h (h = 0, 1) of each panel; element t is operand t / 2, pass
+    // t % 2. The loads are raw buffer loads over the rest of the domain's block
+    // from the panel's first Q row, so rows past naux read as zeros; a column
+    // past nvir gets the offset 2^31, outside every such range, and reads as
+    // zero too. voff: the element's byte offset from the panel's first Q row
+    // (below 48 nov 8 < 2^31: the host keeps nocc nvir under 2^22).
+    const int ac = tid % TILE, qr = tid / TILE;
+    const int64_t col[4] = {(int64_t)it.i * nvir + A0, (int64_t)it.j * nvir + B0,
+                            sq.single ? (int64_t)it.i * nvir + A0 : (int64_t)it.j * nvir + A0,
+                            sq.single ? (int64_t)it.j * nvir + B0 : (int64_t)it.i * nvir + B0};
+    const bool inA = ac < nvir - A0, inB = ac < nvir - B0;
+    uint32_t voff[EPT];
+#pragma unroll
+    for (int t = 0; t < EPT; ++t) {
+        const int op = t / 2, h = t % 2;
+        const int64_t el =
+            (int64_t)(qr + h * RPT + (op >= 2 ? sq.off23 : 0)) * nov + col[op] + ac;
+        voff[t] = ((op & 1) ? inB : inA) ? (uint32_t)(el * 8) : 0x80000000u;
+    }
+    const double *__restrict__ src = (const double *)uniform64((int64_t)(ba.b + dm.b_off));
+
+    double stg[EPT];
+    auto gload = [&](int q0) {
+        const int64_t left = (int64_t)(naux - q0) * nov * 8;
+        const int nrec = left <= 0 ? 0 : (int)std::min<int64_t>(left, 0x7fffffff);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(src + (int64_t)q0 * nov), (short)0, nrec,
+                                              kRawBufferFlags);
+#pragma unroll
+        for (int t = 0; t < EPT; ++t)
+            stg[t] = __builtin_bit_cast(double,
+                                        __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff[t], 0, 0));
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < EPT; ++t) {
+            const int op = t / 2, h = t % 2;
+            lds[buf][op][qr + h * RPT][ac] = stg[t];
         }
+    };
+
+    const int lane = tid % kWave, wid = tid / kWave;
+    const Lane L{A0, B0, wid / 2, wid % 2, lane % 16, lane / 16};
+    bool rv[NB], cv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        rv[k] = A0 + L.wr * TW + k * 16 < dm.nvir_pad;
+        cv[k] = B0 + L.wc * TW + k * 16 < dm.nvir_pad;
+    }
+    bool full = true;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) full = full && rv[k] && cv[k];
+
+    M::acc acc1[NB][NB], acc2[NB][NB];
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            acc1[x][y] = (M::acc){0, 0, 0, 0};
+            acc2[x][y] = (M::acc){0, 0, 0, 0};
+        }
+    // One Q row of the lane's KPL at a time: 4 NB fragments in flight.
+    auto compute = [&](int buf, auto fullv) {
+#pragma unroll
+        for (int s = 0; s < KPL; ++s) {
+            const int qk = L.g * KPL + s;
+            double f0[NB], f1[NB], f2[NB], f3[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                f0[k] = lds[buf][0][qk][L.wr * TW + k * 16 + L.r];
+                f2[k] = lds[buf][2][qk][L.wr * TW + k * 16 + L.r];
+                f1[k] = lds[buf][1][qk][L.wc * TW + k * 16 + L.r];
+                f3[k] = lds[buf][3][qk][L.wc * TW + k * 16 + L.r];
+            }
+#pragma unroll
+            for (int x = 0; x < NB; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+                    if (decltype(fullv)::value || (rv[x] && cv[y])) {
+                        acc1[x][y] = M::mma(f0[x], f1[y], acc1[x][y]);
+                        acc2[x][y] = M::mma(f2[x], f3[y], acc2[x][y]);
+                    }
+        }
+    };
+
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int p = 0; p < sq.iters; ++p) {
+        const int cur = p & 1;
+        if (p + 1 < sq.iters) gload((p + 1) * sq.step);
+        if (full)
+            compute(cur, std::true_type{});
+        else
+            compute(cur, std::false_type{});
+        if (p + 1 < sq.iters) lstore(cur ^ 1);
+        __syncthreads();
+    }
+    if (sq.single) {
+        static_assert(sizeof(lds) >= sizeof(double) * TW * (TILE + 1), "LDS holds a C1 half-tile");
+        derive_c2<TILE, double>(it, L, acc1, acc2, RowOf<double>{}, &lds[0][0][0][0]);
+    }
+    energy_epilogue<TILE>(e, it, L, acc1, acc2, rv, cv, RowOf<double>{}, NoCorr{});
+}
+
+// ---- persistent kernels ------------------------------------------------------------
+// Every workgroup loops until all queues are empty; the item index passes from
+// thread 0 to the others through s_item, written only after the item's own
+// barriers and read after the barrier that ends the previous item.
+// FP64: four workgroups per CU (at most 128 VGPRs, a few spilled in the
+// epilogue; 4 x 37 KB of LDS), measured faster than three without spills.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void
+batched_fp64_kernel(BatchArgs a) {
+    __shared__ __attribute__((aligned(16))) BatchLds64 lds;
+    __shared__ int s_item;
+    EnergyArgs e;
+    energy_args_init(a, Prec::F64, e);
+    int q = blockIdx.x % kXcds, tried = 0;
+    if (threadIdx.x == 0) s_item = next_item(a, q, tried);
+    __syncthreads();
+    for (;;) {
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item < 0) break;
+        int nxt = -1;
+        if (threadIdx.x == 0) nxt = next_item(a, q, tried);
+        const DomDev dm = load_domain(a, item);
+        pair::Item it;
+        item_setup(a, dm, item, it, e);
+        if (dm.tile == 64)
+            fp64_item<64>(a, dm, it, e, lds.t64);
+        else
+            fp64_item<32>(a, dm, it, e, lds.t32);
+        if (threadIdx.x == 0) s_item = nxt;
+        __syncthreads();
+    }
+}
+
+union BatchLdsBf {
+    pair::Bf16Lds<64> t64;
+    pair::Bf16Lds<32> t32;
+};
+
+__global__ __launch_bounds__(256) void batched_bf16_kernel(BatchArgs a) {
+    __shared__ __attribute__((aligned(16))) BatchLdsBf lds;
+    __shared__ int s_item;
+    EnergyArgs e;
+    energy_args_init(a, Prec::BF16X3, e);
+    int q = blockIdx.x % kXcds, tried = 0;
+    if (threadIdx.x == 0) s_item = next_item(a, q, tried);
+    __syncthreads();
+    for (;;) {
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item < 0) break;
+        int nxt = -1;
+        if (threadIdx.x == 0) nxt = next_item(a, q, tried);
+        const DomDev dm = load_domain(a, item);
+        pair::Item it;
+        item_setup(a, dm, item, it, e);
+        e.op0 = (const uint16_t *)a.op0 + dm.bp_off;
+        e.op1 = (const uint16_t *)a.op1 + dm.bp_off;
+        e.diag_corr = a.corr + dm.row_off;
+        if (dm.tile == 64)
+            pair::pair_bf16x3_body<64, false>(e, it, lds.t64);
+        else
+            pair::pair_bf16x3_body<32, false>(e, it, lds.t32);
+        if (threadIdx.x == 0) s_item = nxt;
         __syncthreads();
     }
 }
@@ -226,10 +473,25 @@ __global__ __launch_bounds__(256) void batched_reduce_kernel(BatchArgs a) {
 
 }  // namespace
 
-// ---- graph cache ------------------------------------------------------------------
+// ---- plan and graph cache ------------------------------------------------------------
+struct Plan {
+    std::vector<DomDev> dom;
+    std::vector<int> item_dom;
+    std::vector<PackTile> ptiles;
+    std::vector<int64_t> row_start;
+    int qbegin[kXcds + 1] = {};
+    int64_t nitems = 0, packed = 0, nrows = 0;
+};
+
+// Captured graphs (keyed by everything they bake in) and the plan whose
+// metadata was last uploaded to the batch slot. Deleted whenever a workspace
+// slot moves (reserve) or the tile table changes, which drops both at once.
 struct GraphCache {
     std::map<uint64_t, hipGraphExec_t> execs;
     hipStream_t capture = nullptr;
+    uint64_t plan_key = 0;
+    std::shared_ptr<const Plan> plan;
+    bool meta_on_device = false;
     ~GraphCache() {
         for (auto &kv : execs) (void)hipGraphExecDestroy(kv.second);
         if (capture) (void)hipStreamDestroy(capture);
@@ -250,121 +512,161 @@ static uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603u
     return h;
 }
 
-struct Plan {
-    std::vector<DomDev> dom;
-    std::vector<int> item_dom;
-    std::vector<PackTile> ptiles;
-    std::vector<int64_t> row_start;
-    std::vector<std::pair<int, std::pair<int64_t, int64_t>>> classes;  // tile, [begin, end)
-    int64_t nitems = 0, packed = 0, nrows = 0;
-};
+// FNV-1a over 8-byte words (the descriptor arrays are multiples of 8 bytes).
+static uint64_t fnv1a_words(const void *p, size_t n, uint64_t h) {
+    const unsigned char *c = (const unsigned char *)p;
+    size_t k = 0;
+    for (; k + 8 <= n; k += 8) {
+        uint64_t w;
+        memcpy(&w, c + k, 8);
+        h ^= w;
+        h *= 1099511628211ull;
+    }
+    return fnv1a(c + k, n - k, h);
+}
 
-static fh_status make_plan(const fh_domain *d, size_t ndom, size_t b_len, size_t eps_len,
-                           Prec prec, Plan &pl) {
-    const int kp = panel_depth(prec);
-    pl.dom.resize(ndom);
-    std::vector<int> order(ndom);
+// Q depth of a panel: the FP64 kernel's per tile side, BF16x3's 32.
+static int batch_panel(Prec prec, int tile) {
+    if (prec == Prec::BF16X3) return 32;
+    return tile == 64 ? 8 : 16;
+}
+
+// Tile side of each domain: the tile table's, else 32 up to nvir 40 and 64
+// beyond; a table entry of 128 runs as 64 here.
+static void batch_tiles(Prec prec, const fh_domain *d, size_t ndom, std::vector<int> &tiles) {
+    std::vector<size_t> nv(ndom), na(ndom);
+    for (size_t k = 0; k < ndom; ++k) {
+        nv[k] = d[k].nvir;
+        na[k] = d[k].naux;
+    }
+    tiles.resize(ndom);
+    tuned_tiles(prec, ndom, nv.data(), na.data(), tiles.data());
+    for (size_t k = 0; k < ndom; ++k) {
+        int t = tiles[k];
+        if (t <= 0) t = nv[k] <= 40 ? 32 : 64;
+        tiles[k] = t >= 64 ? 64 : 32;
+    }
+}
+
+static fh_status check_domains(const fh_domain *d, size_t ndom, size_t b_len, size_t eps_len) {
     for (size_t k = 0; k < ndom; ++k) {
         const fh_domain &x = d[k];
         if (x.nocc == 0 || x.nvir == 0 || x.naux == 0)
             return fail(FH_LAYOUT, "domain %zu has an empty dimension (nocc %u nvir %u naux %u)",
                         k, x.nocc, x.nvir, x.naux);
-        if (x.nocc > 4096 || x.nvir > 65536 || x.naux > (1u << 20))
-            return fail(FH_LAYOUT, "domain %zu is beyond the kernel's range", k);
+        if (x.nocc > 4096 || x.nvir > 65536 || x.naux > (1u << 20) ||
+            (uint64_t)x.nocc * x.nvir >= (1u << 22))
+            return fail(FH_LAYOUT,
+                        "domain %zu is beyond the kernel's range (nocc <= 4096, nvir <= 65536, "
+                        "naux <= 2^20, nocc nvir < 2^22)",
+                        k);
         const uint64_t nb = (uint64_t)x.naux * x.nocc * x.nvir;
         if (x.b_offset > b_len || nb > b_len - x.b_offset)
             return fail(FH_LAYOUT, "domain %zu: B [%llu, +%llu) outside the %zu-element buffer", k,
                         (unsigned long long)x.b_offset, (unsigned long long)nb, b_len);
         if (x.eps_offset > eps_len || (uint64_t)x.nocc + x.nvir > eps_len - x.eps_offset)
             return fail(FH_LAYOUT, "domain %zu: eps outside the %zu-element buffer", k, eps_len);
+    }
+    return FH_OK;
+}
+
+static fh_status make_plan(const fh_domain *d, size_t ndom, const std::vector<int> &tiles,
+                           Prec prec, Plan &pl) {
+    pl.dom.resize(ndom);
+    std::vector<double> item_cost(ndom), dom_cost(ndom);
+    for (size_t k = 0; k < ndom; ++k) {
+        const fh_domain &x = d[k];
         DomDev &m = pl.dom[k];
         m.b_off = (int64_t)x.b_offset;
         m.eps_off = (int64_t)x.eps_offset;
         m.nocc = (int)x.nocc;
         m.nvir = (int)x.nvir;
         m.naux = (int)x.naux;
+        m.tile = tiles[k];
+        const int kp = batch_panel(prec, m.tile);
         m.nvir_pad = (m.nvir + 15) / 16 * 16;
         m.naux_pad = (m.naux + kp - 1) / kp * kp;
-        int tile = tuned_tile(prec, x.nvir, x.naux);
-        if (tile <= 0) tile = x.nvir <= 40 ? 32 : 64;
-        m.tile = tile;
-        m.ntile = (m.nvir_pad + tile - 1) / tile;
+        m.ntile = (m.nvir_pad + m.tile - 1) / m.tile;
         m.ntp = m.ntile * (m.ntile + 1) / 2;
-        order[k] = (int)k;
-    }
-    // Items grouped by tile class; inside a class, most expensive domains first
-    // (cost of an item ~ naux_pad * tile^2, times the domain's item count).
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        const DomDev &a = pl.dom[x], &b = pl.dom[y];
-        if (a.tile != b.tile) return a.tile > b.tile;
-        return (double)a.naux_pad > (double)b.naux_pad;
-    });
-    int64_t item = 0, packed = 0, rows = 0;
-    int cur_tile = -1;
-    for (int k : order) {
-        DomDev &m = pl.dom[k];
-        if (m.tile != cur_tile) {
-            if (cur_tile > 0) pl.classes.back().second.second = item;
-            pl.classes.push_back({m.tile, {item, item}});
-            cur_tile = m.tile;
-        }
-        m.item0 = item;
         const int64_t n = (int64_t)m.nocc * (m.nocc + 1) / 2 * m.ntp;
-        pl.item_dom.insert(pl.item_dom.end(), (size_t)n, k);
-        item += n;
+        item_cost[k] = (double)m.naux_pad * m.tile * m.tile;
+        dom_cost[k] = item_cost[k] * (double)n;
     }
-    if (!pl.classes.empty()) pl.classes.back().second.second = item;
+    // Deal the domains to the XCD queues, largest first, each to the queue
+    // with the least work so far; inside a queue, largest items first.
+    std::vector<int> order(ndom);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return dom_cost[x] > dom_cost[y]; });
+    std::vector<std::vector<int>> queue(kXcds);
+    double load[kXcds] = {};
+    for (int k : order) {
+        const int qx = (int)(std::min_element(load, load + kXcds) - load);
+        queue[qx].push_back(k);
+        load[qx] += dom_cost[k];
+    }
+    int64_t item = 0;
+    for (int qx = 0; qx < kXcds; ++qx) {
+        std::stable_sort(queue[qx].begin(), queue[qx].end(),
+                         [&](int x, int y) { return item_cost[x] > item_cost[y]; });
+        pl.qbegin[qx] = (int)std::min<int64_t>(item, 0x7fffffff);
+        for (int k : queue[qx]) {
+            DomDev &m = pl.dom[k];
+            m.item0 = item;
+            const int64_t n = (int64_t)m.nocc * (m.nocc + 1) / 2 * m.ntp;
+            if (item + n > 0x7fffffffLL - 65536)
+                return fail(FH_LAYOUT, "too many work items (more than %lld)",
+                            (long long)(item + n));
+            pl.item_dom.insert(pl.item_dom.end(), (size_t)n, k);
+            item += n;
+        }
+    }
+    pl.qbegin[kXcds] = (int)item;
     pl.nitems = item;
-    // Packed offsets in descriptor order (row_start must be monotone in d).
-    pl.row_start.resize(ndom + 1);
-    for (size_t k = 0; k < ndom; ++k) {
-        DomDev &m = pl.dom[k];
-        m.bp_off = packed;
-        m.row_off = rows;
-        pl.row_start[k] = rows;
-        const int64_t nr = (int64_t)m.nocc * m.nvir_pad;
-        packed += nr * m.naux_pad;
-        rows += nr;
-        for (int64_t r0 = 0; r0 < nr; r0 += 64)
-            for (int q0 = 0; q0 < m.naux_pad; q0 += 64)
-                pl.ptiles.push_back(PackTile{(int)k, (int)r0, q0, 0});
+    // BF16x3 packed offsets in descriptor order (row_start must be monotone in d).
+    if (prec == Prec::BF16X3) {
+        int64_t packed = 0, rows = 0;
+        pl.row_start.resize(ndom + 1);
+        for (size_t k = 0; k < ndom; ++k) {
+            DomDev &m = pl.dom[k];
+            m.bp_off = packed;
+            m.row_off = rows;
+            pl.row_start[k] = rows;
+            const int64_t nr = (int64_t)m.nocc * m.nvir_pad;
+            packed += nr * m.naux_pad;
+            rows += nr;
+            for (int64_t r0 = 0; r0 < nr; r0 += 64)
+                for (int q0 = 0; q0 < m.naux_pad; q0 += 64)
+                    pl.ptiles.push_back(PackTile{(int)k, (int)r0, q0, 0});
+        }
+        pl.row_start[ndom] = rows;
+        pl.packed = packed;
+        pl.nrows = rows;
     }
-    pl.row_start[ndom] = rows;
-    pl.packed = packed;
-    pl.nrows = rows;
-    if (pl.nitems > 0x7fffffffLL) return fail(FH_LAYOUT, "too many work items (%lld)",
-                                              (long long)pl.nitems);
     return FH_OK;
 }
 
+// Persistent grid: as many workgroups as fit on the chip at once (256 CUs;
+// four per CU for FP64, two for BF16x3 by its LDS), never more than the items.
+static int64_t batch_grid(Prec prec, int64_t nitems) {
+    return std::min<int64_t>(nitems, prec == Prec::BF16X3 ? 512 : 1024);
+}
+
 static void enqueue(const BatchArgs &ba, const Plan &pl, Prec prec, hipStream_t st) {
-    const int ncls = (int)pl.classes.size();
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, ba.counters, ncls);
-    const unsigned pblocks = (unsigned)std::min<int64_t>(std::max<int64_t>(pl.ptiles.size(), 1),
-                                                         4096);
-    hipLaunchKernelGGL(batched_pack_kernel, dim3(pblocks), dim3(256), 0, st, ba);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, ba.counters, kXcds);
     if (prec == Prec::BF16X3) {
+        const unsigned pblocks =
+            (unsigned)std::min<int64_t>(std::max<int64_t>(pl.ptiles.size(), 1), 4096);
+        hipLaunchKernelGGL(batched_pack_kernel, dim3(pblocks), dim3(256), 0, st, ba);
         const unsigned rb = (unsigned)std::min<int64_t>((pl.nrows + 3) / 4, 8192);
         hipLaunchKernelGGL(batched_lo_sq_kernel, dim3(std::max(rb, 1u)), dim3(256), 0, st, ba);
     }
-    for (int c = 0; c < ncls; ++c) {
-        const int tile = pl.classes[c].first;
-        const int64_t b0 = pl.classes[c].second.first, b1 = pl.classes[c].second.second;
-        // Persistent grid: a few workgroups per CU, never more than the items.
-        const int64_t grid = std::min<int64_t>(b1 - b0, tile == 128 ? 256 : tile == 64 ? 768 : 1536);
-        if (grid <= 0) continue;
-        const bool bf = prec == Prec::BF16X3;
-#define FH_BLAUNCH(T, B)                                                                     \
-    hipLaunchKernelGGL((batched_pair_kernel<T, B>), dim3((unsigned)grid), dim3(256), 0, st, ba, \
-                       b0, b1, c)
-        if (tile == 32) {
-            if (bf) FH_BLAUNCH(32, true); else FH_BLAUNCH(32, false);
-        } else if (tile == 64) {
-            if (bf) FH_BLAUNCH(64, true); else FH_BLAUNCH(64, false);
-        } else {
-            if (bf) FH_BLAUNCH(128, true); else FH_BLAUNCH(128, false);
-        }
-#undef FH_BLAUNCH
+    const int64_t grid = batch_grid(prec, pl.nitems);
+    if (grid > 0) {
+        if (prec == Prec::BF16X3)
+            hipLaunchKernelGGL(batched_bf16_kernel, dim3((unsigned)grid), dim3(256), 0, st, ba);
+        else
+            hipLaunchKernelGGL(batched_fp64_kernel, dim3((unsigned)grid), dim3(256), 0, st, ba);
     }
     hipLaunchKernelGGL(batched_reduce_kernel, dim3((unsigned)pl.dom.size()), dim3(256), 0, st, ba);
 }
@@ -394,47 +696,82 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
     if (!domains || !b_dev || !eps_dev || !e_os || !e_ss)
         return fail(FH_LAYOUT, "null pointer argument");
     if (ndom > (1u << 20)) return fail(FH_LAYOUT, "too many domains (%zu)", ndom);
-    Plan pl;
-    fh_status st = make_plan(domains, ndom, b_len, eps_len, prec, pl);
+    fh_status st = check_domains(domains, ndom, b_len, eps_len);
     if (st != FH_OK) return st;
     st = ensure_device(s);
     if (st != FH_OK) return st;
+    // The plan is a function of the descriptors, the tile of each domain and
+    // the precision; reuse the last one (and its uploaded metadata) when they
+    // match.
+    std::vector<int> tiles;
+    batch_tiles(prec, domains, ndom, tiles);
+    uint64_t pkey = fnv1a_words(domains, sizeof(fh_domain) * ndom, 1469598103934665603ull);
+    pkey = fnv1a_words(tiles.data(), sizeof(int) * ndom, pkey);
+    pkey = fnv1a(&prec, sizeof prec, pkey);
+    std::shared_ptr<const Plan> plan;
+    if (s.graphs && s.graphs->plan && s.graphs->plan_key == pkey) {
+        plan = s.graphs->plan;
+    } else {
+        auto np = std::make_shared<Plan>();
+        st = make_plan(domains, ndom, tiles, prec, *np);
+        if (st != FH_OK) return st;
+        plan = np;
+    }
+    const Plan &pl = *plan;
     const bool bf = prec == Prec::BF16X3;
-    const size_t eb = bf ? 2 : 8;
     const size_t meta = sizeof(DomDev) * ndom + sizeof(int) * pl.item_dom.size() +
-                        sizeof(PackTile) * pl.ptiles.size() + sizeof(int64_t) * (ndom + 1) + 256;
+                        sizeof(PackTile) * pl.ptiles.size() + sizeof(int64_t) * pl.row_start.size() +
+                        256;
     std::vector<std::pair<int, size_t>> req = {
         {kSlotBatch, meta},
-        {kSlotOp0, pl.packed * eb},
         {kSlotItems, sizeof(double) * 4 * pl.nitems},
         {kSlotBatchOut, sizeof(double) * 2 * ndom + 64 * sizeof(int)},
     };
     if (bf) {
-        req.push_back({kSlotOp1, pl.packed * eb});
+        req.push_back({kSlotOp0, pl.packed * 2});
+        req.push_back({kSlotOp1, pl.packed * 2});
         req.push_back({kSlotDiag, sizeof(double) * pl.nrows});
     }
-    st = reserve(s, req);
+    st = reserve(s, req);   // a slot that moves drops the graph cache (and the plan's upload)
     if (st != FH_OK) return st;
+    if (!s.graphs) s.graphs = new GraphCache();
+    GraphCache &gc = *s.graphs;
+    if (gc.plan_key != pkey || gc.plan != plan) {
+        gc.plan_key = pkey;
+        gc.plan = plan;
+        gc.meta_on_device = false;
+    }
     // Metadata: [DomDev...][item_dom...][PackTile...][row_start...]
     char *mbase = (char *)s.slots[kSlotBatch].ptr;
-    std::vector<char> host(meta, 0);
     size_t off = 0;
-    auto put = [&](const void *p, size_t n, size_t align) {
+    auto place = [&](size_t n, size_t align) {
         off = (off + align - 1) / align * align;
-        memcpy(host.data() + off, p, n);
         const size_t at = off;
         off += n;
         return at;
     };
-    const size_t o_dom = put(pl.dom.data(), sizeof(DomDev) * ndom, 16);
-    const size_t o_item = put(pl.item_dom.data(), sizeof(int) * pl.item_dom.size(), 16);
-    const size_t o_pt = put(pl.ptiles.data(), sizeof(PackTile) * pl.ptiles.size(), 16);
-    const size_t o_rs = put(pl.row_start.data(), sizeof(int64_t) * (ndom + 1), 16);
+    const size_t o_dom = place(sizeof(DomDev) * ndom, 16);
+    const size_t o_item = place(sizeof(int) * pl.item_dom.size(), 16);
+    const size_t o_pt = place(sizeof(PackTile) * pl.ptiles.size(), 16);
+    const size_t o_rs = place(sizeof(int64_t) * pl.row_start.size(), 16);
     hipStream_t ust = (hipStream_t)stream;
-    st = hip_check(hipMemcpyAsync(mbase, host.data(), off, hipMemcpyHostToDevice, ust),
-                   "H2D batch metadata");
-    if (st != FH_OK) return st;
-    s.pending.h2d_bytes += off;
+    if (!gc.meta_on_device) {
+        std::vector<char> host(off, 0);
+        memcpy(host.data() + o_dom, pl.dom.data(), sizeof(DomDev) * ndom);
+        memcpy(host.data() + o_item, pl.item_dom.data(), sizeof(int) * pl.item_dom.size());
+        if (!pl.ptiles.empty())
+            memcpy(host.data() + o_pt, pl.ptiles.data(), sizeof(PackTile) * pl.ptiles.size());
+        if (!pl.row_start.empty())
+            memcpy(host.data() + o_rs, pl.row_start.data(), sizeof(int64_t) * pl.row_start.size());
+        st = hip_check(hipMemcpyAsync(mbase, host.data(), off, hipMemcpyHostToDevice, ust),
+                       "H2D batch metadata");
+        if (st != FH_OK) return st;
+        // The staging vector is freed on return: let the copy finish first.
+        st = hip_check(hipStreamSynchronize(ust), "sync after metadata upload");
+        if (st != FH_OK) return st;
+        s.pending.h2d_bytes += off;
+        gc.meta_on_device = true;
+    }
 
     BatchArgs ba;
     ba.prec = prec;
@@ -444,7 +781,7 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
     ba.item_dom = (const int *)(mbase + o_item);
     ba.ptiles = (const PackTile *)(mbase + o_pt);
     ba.nptiles = (int64_t)pl.ptiles.size();
-    ba.op0 = s.slots[kSlotOp0].ptr;
+    ba.op0 = bf ? s.slots[kSlotOp0].ptr : nullptr;
     ba.op1 = bf ? s.slots[kSlotOp1].ptr : nullptr;
     ba.corr = bf ? (double *)s.slots[kSlotDiag].ptr : nullptr;
     ba.nrows = pl.nrows;
@@ -453,19 +790,18 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
     ba.item_partials = (double *)s.slots[kSlotItems].ptr;
     ba.out = (double *)s.slots[kSlotBatchOut].ptr;
     ba.counters = (int *)(ba.out + 2 * ndom);
+    for (int x = 0; x <= kXcds; ++x) ba.qbegin[x] = pl.qbegin[x];
 
+    // Timing: the whole launch sequence (packing included) is the contract phase.
     mark(s, 0, ust);
+    mark(s, 1, ust);
     if (use_graph) {
-        if (!s.graphs) s.graphs = new GraphCache();
-        GraphCache &gc = *s.graphs;
-        // Key: everything the captured launches bake in -- the resolved plan
-        // (shapes, offsets, tile choices) and every device pointer.
-        uint64_t key = fnv1a(pl.dom.data(), sizeof(DomDev) * ndom);
+        // Key: everything the captured launches bake in -- the plan and every
+        // device pointer.
+        uint64_t key = pkey;
         const void *ptrs[8] = {b_dev, eps_dev, ba.op0, ba.op1, ba.item_partials, mbase, ba.corr,
                                ba.out};
         key = fnv1a(ptrs, sizeof ptrs, key);
-        key = fnv1a(&prec, sizeof prec, key);
-        key = fnv1a(&pl.nitems, sizeof pl.nitems, key);
         auto itg = gc.execs.find(key);
         if (itg == gc.execs.end()) {
             if (!gc.capture) {
@@ -473,7 +809,8 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
                                "create capture stream");
                 if (st != FH_OK) return st;
             }
-            // The metadata upload above must land before the graph first runs.
+            // Work already queued on the caller's stream must land before the
+            // graph first runs.
             st = hip_check(hipStreamSynchronize(ust), "sync before capture");
             if (st != FH_OK) return st;
             st = hip_check(hipStreamBeginCapture(gc.capture, hipStreamCaptureModeThreadLocal),
@@ -496,7 +833,6 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
         hipError_t he = hipGetLastError();
         if (he != hipSuccess) return fail(FH_KERNEL, "batched kernels: %s", hipGetErrorString(he));
     }
-    mark(s, 1, ust);
     mark(s, 2, ust);
     std::vector<double> res(2 * ndom);
     st = hip_check(hipMemcpyAsync(res.data(), ba.out, sizeof(double) * res.size(),
@@ -507,8 +843,8 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
     st = hip_check(hipStreamSynchronize(ust), "stream sync");
     if (st != FH_OK) return st;
     s.pending.d2h_bytes += sizeof(double) * res.size();
-    s.pending.kernel_launches += 3 + pl.classes.size();
-    finish_timing(s, (uint64_t)pl.nitems, pl.classes.size() == 1 ? pl.classes[0].first : 0);
+    s.pending.kernel_launches += bf ? 5 : 3;
+    finish_timing(s, (uint64_t)pl.nitems, 0);
     for (size_t k = 0; k < ndom; ++k) {
         e_os[k] = res[2 * k];
         e_ss[k] = res[2 * k + 1];

@@ -113,6 +113,8 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t dep
                       hipStream_t stream);
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec);
 int tuned_tile(Prec prec, size_t nvir, size_t naux);
+// tuned_tile of n problems at once (one environment read, one lock).
+void tuned_tiles(Prec prec, size_t n, const size_t *nvir, const size_t *naux, int *out);
 size_t packed_elem_bytes(Prec p);
 // The device's rocBLAS handle (created on first use, atomics off so that its
 // GEMMs are deterministic), bound to stream.

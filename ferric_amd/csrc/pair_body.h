@@ -83,6 +83,10 @@ __device__ __forceinline__ double damping(bool kap, double kappa, double d) {
     return d1 * d1;
 }
 
+struct NoCorr {
+    __device__ double operator()(int, int) const { return 0.0; }
+};
+
 // The lane's coordinates inside the tile pair.
 struct Lane {
     int A0, B0, wr, wc, r, g;
@@ -90,6 +94,10 @@ struct Lane {
 
 // Energy epilogue: acc1/acc2 hold g_ab / g_ba; row_of maps (g, v) to the row
 // in a 16x16 block; corr(i, a) is added to K_ii[a, a] (0 except for BF16x3).
+// The lane's virtual energies are read ahead of the arithmetic that uses
+// them, a row block at a time (indices clamped into range; elements past nvir
+// are skipped below), so their latencies overlap instead of following one
+// another through the element loop.
 template <int TILE, typename Acc, typename RowOfT, typename Corr>
 __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item &it, const Lane &L,
                                                 Acc (&acc1)[TILE / 32][TILE / 32],
@@ -97,27 +105,37 @@ __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item 
                                                 const bool *rv, const bool *cv, RowOfT row_of,
                                                 Corr corr) {
     constexpr int TW = TILE / 2, NB = TW / 16;
+    constexpr bool kCorr = !std::is_same<Corr, NoCorr>::value;
     const double eij = a.eps_occ[it.i] + a.eps_occ[it.j];
     const bool kap = !isnan(a.kappa);
+    const int last = a.nvir - 1;
+    double eb[NB];
+#pragma unroll
+    for (int y = 0; y < NB; ++y) eb[y] = a.eps_vir[min(L.B0 + L.wc * TW + y * 16 + L.r, last)];
     KSum os, ss;
 #pragma unroll
-    for (int x = 0; x < NB; ++x)
+    for (int x = 0; x < NB; ++x) {
+        double ea[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            ea[v] = a.eps_vir[min(L.A0 + L.wr * TW + x * 16 + row_of(L.g, v), last)];
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
             if (!(rv[x] && cv[y])) continue;
             const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
             if (bb >= a.nvir) continue;
-            const double eb = a.eps_vir[bb];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
                 if (aa >= a.nvir) continue;
-                const double d = eij - a.eps_vir[aa] - eb;
+                const double d = eij - ea[v] - eb[y];
                 double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
-                if (it.i == it.j && aa == bb) {
-                    const double c = corr(it.i, aa);
-                    g1 += c;
-                    g2 += c;
+                if constexpr (kCorr) {
+                    if (it.i == it.j && aa == bb) {
+                        const double c = corr(it.i, aa);
+                        g1 += c;
+                        g2 += c;
+                    }
                 }
                 const double inv = 1.0 / d;
                 const double dd = g1 - g2;
@@ -126,6 +144,7 @@ __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item 
                 ss.add(dmp * ((dd * dd) * inv));
             }
         }
+    }
     const double w = (it.i == it.j ? 1.0 : 2.0) * (it.ta == it.tb ? 0.5 : 1.0);
     block_store(os, ss, w, a.item_partials + 4 * it.id);
 }
@@ -371,9 +390,6 @@ __device__ __forceinline__ void amp_store(const EnergyArgs &a, const Item &it, c
         }
 }
 
-struct NoCorr {
-    __device__ double operator()(int, int) const { return 0.0; }
-};
 struct LoSqCorr {
     const double *p;
     int nvir_pad;
@@ -520,6 +536,7 @@ __device__ __forceinline__ void pair_k4_body(const EnergyArgs &a, const Item &it
     constexpr int NCH = 4 * OPCH;         // chunks per panel (4 operands)
     constexpr int CPT = (NCH + 255) / 256;
     constexpr int LDR = KP + EPC;         // LDS row stride: one chunk of padding
+    static_assert(sizeof(lds[0][0][0]) == LDR * sizeof(T), "K4Lds rows carry one spare chunk");
     constexpr int KPL = KP / 4;           // k values per lane per panel
     static_assert((KPL * sizeof(T)) % 16 == 0, "fragments are whole 16-byte chunks");
     typedef T frag_t __attribute__((ext_vector_type(KPL)));

@@ -23,17 +23,35 @@ std::mutex g_tune_mu;
 std::vector<TuneEntry> g_table;
 }  // namespace
 
-int tuned_tile(Prec prec, size_t nvir, size_t naux) {
+static int forced_tile() {
     const char *env = getenv("FERRIC_HIP_TILE");
     if (env && *env) {
-        int t = atoi(env);
+        const int t = atoi(env);
         if (t == 32 || t == 64 || t == 128) return t;
     }
-    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return 0;
+}
+
+static int table_tile(Prec prec, size_t nvir, size_t naux) {
     // First entry (in installation order) whose bucket holds the shape.
     for (const auto &e : g_table)
         if (e.prec == (int)prec && nvir <= e.nvir_max && naux <= e.naux_max) return e.tile;
     return 0;
+}
+
+int tuned_tile(Prec prec, size_t nvir, size_t naux) {
+    if (const int t = forced_tile()) return t;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return table_tile(prec, nvir, naux);
+}
+
+void tuned_tiles(Prec prec, size_t n, const size_t *nvir, const size_t *naux, int *out) {
+    if (const int t = forced_tile()) {
+        for (size_t k = 0; k < n; ++k) out[k] = t;
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (size_t k = 0; k < n; ++k) out[k] = table_tile(prec, nvir[k], naux[k]);
 }
 
 void tune_set(int prec, size_t nvir_max, size_t naux_max, int tile) {

@@ -72,6 +72,32 @@ def test_pno_entry_checks_before_the_device():
     assert call([(-1, 1)], 0.0) == _lib.Status.LAYOUT
 
 
+def test_batched_descriptor_checks_before_the_device():
+    """Malformed domain descriptors are FH_LAYOUT before any device work: an
+    empty dimension, a block past the buffer, eps past its buffer, and a
+    domain beyond the kernel's range (nocc nvir >= 2^22)."""
+    from ferric_amd.batched import descriptors
+
+    L = _lib.lib()
+    eo, es = np.zeros(1), np.zeros(1)
+    fake = ctypes.c_void_p(0x1000)   # never dereferenced: the checks come first
+
+    def call(nocc, nvir, naux, b_len, eps_len, b_off=0, e_off=0):
+        d = descriptors([nocc], [nvir], [naux], [b_off], [e_off])
+        return L.fh_rimp2_batched_dev(d, 1, fake, b_len, fake, eps_len,
+                                      int(_lib.Precision.FP64), 0, None,
+                                      eo.ctypes.data_as(_lib._c_dp), es.ctypes.data_as(_lib._c_dp))
+
+    assert call(0, 5, 8, 100, 100) == _lib.Status.LAYOUT
+    assert call(2, 5, 8, 79, 100) == _lib.Status.LAYOUT
+    assert call(2, 5, 8, 80, 6) == _lib.Status.LAYOUT
+    assert call(2, 5, 8, 80, 7, e_off=1) == _lib.Status.LAYOUT
+    assert call(4096, 1024, 1, 1 << 22, 5120) == _lib.Status.LAYOUT
+    assert call(2, 5, 8, 80, 7, b_off=1) == _lib.Status.LAYOUT
+    with pytest.raises(_lib.LayoutError):
+        descriptors([2], [-5], [8], [0], [0])
+
+
 def test_from_g_entries_check_before_the_device():
     eps = np.array([-1.0, -0.8, 0.5, 0.6, 0.7])
     g = np.zeros((6, 6))

@@ -1,9 +1,10 @@
-"""Batched local-MP2 domains (one persistent launch per tile class), run with -m gpu.
+"""Batched local-MP2 domains (one persistent launch over every tile side), run with -m gpu.
 
 Every domain is an independent RI-MP2 problem, so each domain's energy is checked
 against the oracle (FP64: 1e-10 Eh; BF16x3: the exact-format energy within the
 FP32-accumulation bound). hipGraph replays are compared with the eager result
-after every replay, with eager work in between.
+after every replay, with eager work in between. One launch that mixes 32- and
+64-wide tiles is checked against the oracle too.
 """
 import numpy as np
 import pytest
@@ -76,7 +77,7 @@ def test_graph_replays_match_eager(small_set, small_dev, prec):
         assert np.array_equal(again[0], eager[0])
 
 
-@pytest.mark.parametrize("tile", ["32", "64", "128"])
+@pytest.mark.parametrize("tile", ["32", "64", "128"])   # 128 runs as 64 in the batched kernel
 def test_every_tile_class(small_set, small_dev, oracle_mod, tile, monkeypatch):
     monkeypatch.setenv("FERRIC_HIP_TILE", tile)
     ds = small_set
@@ -86,6 +87,36 @@ def test_every_tile_class(small_set, small_dev, oracle_mod, tile, monkeypatch):
         p = ds.domain(d)
         want = oracle_mod.spin_components(p.b_ov, p.eps, p.nocc, p.nvir)
         assert abs(e_os[d] - want[0]) <= 1e-10 and abs(e_ss[d] - want[1]) <= 1e-10, (tile, d)
+
+
+@pytest.mark.parametrize("prec", [Precision.FP64, Precision.BF16X3])
+def test_mixed_tile_sides_in_one_launch(small_set, small_dev, oracle_mod, prec):
+    from ferric_amd import autotune
+
+    ds = small_set
+    assert ds.nvir.min() <= 60 < ds.nvir.max()
+    try:
+        autotune.install([{"precision": prec.name, "nvir_max": 60, "naux_max": 1 << 20,
+                           "tile": 32},
+                          {"precision": prec.name, "nvir_max": 1 << 16, "naux_max": 1 << 20,
+                           "tile": 64}])
+        e_os, e_ss = domain_set_energies(ds, prec, use_graph=True, b_dev=small_dev[0],
+                                         eps_dev=small_dev[1])
+    finally:
+        autotune.install(autotune.load_table())
+    print("mixed tiles", prec.name, flush=True)
+    for d in range(len(ds)):
+        p = ds.domain(d)
+        if prec == Precision.FP64:
+            want = oracle_mod.spin_components(p.b_ov, p.eps, p.nocc, p.nvir)
+            assert abs(e_os[d] - want[0]) <= 1e-10 and abs(e_ss[d] - want[1]) <= 1e-10, d
+        else:
+            hi, lo = quant.bf16_split(p.b_ov)
+            g = hi.T @ hi + hi.T @ lo + lo.T @ hi + np.diag(np.sum(lo * lo, axis=0))
+            want = oracle_mod.spin_components_from_g(g, p.eps, p.nocc, p.nvir)
+            bnd = oracle_mod.energy_bound(hi + lo, p.eps, p.nocc, p.nvir,
+                                          eps_g=(p.naux + 16) * U32)
+            assert abs(e_os[d] - want[0]) <= bnd[0] and abs(e_ss[d] - want[1]) <= bnd[1], d
 
 
 def test_full_512_domains_against_single_problem_path():
