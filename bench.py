@@ -155,7 +155,7 @@ def batched_leg(args, dev):
     for prec in (Precision.FP64, Precision.BF16X3):
         domain_set_energies(ds, prec, True, bd, ed)   # capture + warm
         torch.cuda.synchronize()
-        k = max(10, args.steps)
+        k = max(30, args.steps)
         # Wall time per call with the library's event timing off, then the
         # device time of the launch sequence from a second run with it on.
         set_timing(False)

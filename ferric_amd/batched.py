@@ -1,11 +1,11 @@
-"""Batched local-MP2 pair domains on the device (one persistent launch per tile class).
+"""Batched local-MP2 pair domains on the device (one persistent launch over all of them).
 
 Each domain is an independent closed-shell RI-MP2 pair-energy problem: its B
 (row-major (naux_d, nocc_d*nvir_d)) lies at ``b_offset`` of one flat FP64 buffer
 and its orbital energies [occupied..., virtual...] at ``eps_offset`` of another.
 The reference walks its pair domains serially on the host
-(crates/ferric-mp2/src/dlpno_mp2.rs:718-745); here they share one device-wide
-work queue. ``use_graph`` captures the launch sequence in a hipGraph on first
+(crates/ferric-mp2/src/dlpno_mp2.rs:718-745); here they share one persistent
+launch that pulls work items from per-XCD queues. ``use_graph`` captures the launch sequence in a hipGraph on first
 use for a descriptor set and replays it afterwards.
 """
 from __future__ import annotations

@@ -66,8 +66,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, zip(srcs, objs)))
     tmp = LIB + ".tmp"
-    # rocBLAS carries the plain dense GEMMs of the dressing stage, the INT8X4
-    # adjoint's gamma and the PNO transforms; rocSOLVER the PNO eigensolves.
+    # rocBLAS carries the plain dense GEMMs of the dressing stage and the PNO
+    # transforms; rocSOLVER the PNO eigensolves and the domain fits' Cholesky.
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)

@@ -259,8 +259,9 @@ __device__ __forceinline__ void fp64_item(const BatchArgs &ba, const DomDev &dm,
     constexpr int KP = QPanel<TILE>::KP, KPL = KP / 4;
     constexpr int TW = TILE / 2, NB = TW / 16;
     constexpr int RPT = 256 / TILE;            // Q rows per pass of the workgroup
-    constexpr int EPT = 8;                     // elements per thread per panel
-    static_assert(KP == 2 * RPT, "two passes per operand");
+    constexpr int NPASS = KP / RPT;            // passes per operand
+    constexpr int EPT = 4 * NPASS;             // elements per thread per panel
+    static_assert(KP % RPT == 0, "whole passes per operand");
     using M = Mfma4<double>;
 
     const int tid = threadIdx.x;
@@ -269,8 +270,8 @@ __device__ __forceinline__ void fp64_item(const BatchArgs &ba, const DomDev &dm,
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
     const SplitQ sq(it, dm.naux_pad, KP);
     // Thread tid loads column a = tid % TILE of every operand, in Q rows
-    // qr + RPT h (h = 0, 1) of each panel; element t is operand t / 2, pass
-    // t % 2. The loads are raw buffer loads over the rest of the domain's block
+    // qr + RPT h (h < NPASS) of each panel; element t is operand t / NPASS,
+    // pass t % NPASS. The loads are raw buffer loads over the rest of the domain's block
     // from the panel's first Q row, so rows past naux read as zeros; a column
     // past nvir gets the offset 2^31, outside every such range, and reads as
     // zero too. voff: the element's byte offset from the panel's first Q row
@@ -283,7 +284,7 @@ __device__ __forceinline__ void fp64_item(const BatchArgs &ba, const DomDev &dm,
     uint32_t voff[EPT];
 #pragma unroll
     for (int t = 0; t < EPT; ++t) {
-        const int op = t / 2, h = t % 2;
+        const int op = t / NPASS, h = t % NPASS;
         const int64_t el =
             (int64_t)(qr + h * RPT + (op >= 2 ? sq.off23 : 0)) * nov + col[op] + ac;
         voff[t] = ((op & 1) ? inB : inA) ? (uint32_t)(el * 8) : 0x80000000u;
@@ -305,7 +306,7 @@ __device__ __forceinline__ void fp64_item(const BatchArgs &ba, const DomDev &dm,
     auto lstore = [&](int buf) {
 #pragma unroll
         for (int t = 0; t < EPT; ++t) {
-            const int op = t / 2, h = t % 2;
+            const int op = t / NPASS, h = t % NPASS;
             lds[buf][op][qr + h * RPT][ac] = stg[t];
         }
     };
@@ -330,7 +331,9 @@ __device__ __forceinline__ void fp64_item(const BatchArgs &ba, const DomDev &dm,
             acc1[x][y] = (M::acc){0, 0, 0, 0};
             acc2[x][y] = (M::acc){0, 0, 0, 0};
         }
-    // One Q row of the lane's KPL at a time: 4 NB fragments in flight.
+    // One Q row of the lane's KPL at a time: 4 NB fragments in flight. (An
+    // MFMA order that alternates C1 and C2 of one block over the panel's KPL
+    // steps, which a register-only loop sustains faster, measured slower here.)
     auto compute = [&](int buf, auto fullv) {
 #pragma unroll
         for (int s = 0; s < KPL; ++s) {
@@ -528,7 +531,7 @@ static uint64_t fnv1a_words(const void *p, size_t n, uint64_t h) {
 // Q depth of a panel: the FP64 kernel's per tile side, BF16x3's 32.
 static int batch_panel(Prec prec, int tile) {
     if (prec == Prec::BF16X3) return 32;
-    return tile == 64 ? 8 : 16;
+    return tile == 64 ? QPanel<64>::KP : QPanel<32>::KP;
 }
 
 // Tile side of each domain: the tile table's, else 32 up to nvir 40 and 64

@@ -292,7 +292,9 @@ fh_status fh_rimp2_mp2_density_dev(const void *b_dev, fh_dtype b_dtype, size_t n
  * domains is a HOST array; b_dev (FP64) and eps_dev (FP64) are the packed
  * device buffers the descriptors point into; e_os/e_ss are HOST arrays of
  * ndom. precision: FH_FP64 or FH_BF16X3. use_graph != 0 captures the launch
- * sequence in a hipGraph on first use for this descriptor set and replays it. */
+ * sequence in a hipGraph on first use for this descriptor set and replays it.
+ * A domain must have nocc <= 4096, nvir <= 65536, naux <= 2^20 and
+ * nocc nvir < 2^22 (FH_LAYOUT otherwise). */
 fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom, const double *b_dev,
                                size_t b_len, const double *eps_dev, size_t eps_len,
                                fh_precision precision, int use_graph, void *stream,
