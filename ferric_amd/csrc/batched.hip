@@ -230,9 +230,6 @@ __device__ __forceinline__ void energy_args_init(const BatchArgs &a, Prec prec, 
 // B[k][col r] for the KPL Q rows k = g KPL + s of the panel, so a half-wave
 // reads 16 consecutive doubles in two Q rows KPL apart; KPL * LDA * 8 = 1152
 // = 128 (mod 256) bytes puts them on disjoint halves of the banks.
-// Dword 3 of a raw (untyped, stride 0) gfx9 buffer resource.
-constexpr int kRawBufferFlags = 0x00020000;
-
 template <int TILE>
 struct QPanel;
 template <>
@@ -293,11 +290,8 @@ __device__ __forceinline__ void fp64_item(const BatchArgs &ba, const DomDev &dm,
 
     double stg[EPT];
     auto gload = [&](int q0) {
-        const int64_t left = (int64_t)(naux - q0) * nov * 8;
-        const int nrec = left <= 0 ? 0 : (int)std::min<int64_t>(left, 0x7fffffff);
         const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void *)(src + (int64_t)q0 * nov), (short)0, nrec,
-                                              kRawBufferFlags);
+            raw_rsrc(src + (int64_t)q0 * nov, (int64_t)(naux - q0) * nov * 8);
 #pragma unroll
         for (int t = 0; t < EPT; ++t)
             stg[t] = __builtin_bit_cast(double,

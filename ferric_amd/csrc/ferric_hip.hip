@@ -238,8 +238,8 @@ fh_status check_kappa(double kappa, bool *has) {
 // (193 vs 204 ms).
 int choose_tile(size_t nvir, Prec prec) {
     if (nvir <= 24) return 32;
-    if (nvir <= 80 || prec != Prec::BF16X3) return 64;
-    return 128;
+    if (nvir <= 80 || (prec != Prec::BF16X3 && prec != Prec::FP8)) return 64;
+    return 128;  // the eight-wave BF16x3 and FP8 kernels
 }
 
 size_t packed_elem_bytes(Prec p) {

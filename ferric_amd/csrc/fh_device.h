@@ -14,6 +14,16 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned short v8u16 __attribute__((ext_vector_type(8)));
 
 constexpr int kWave = 64;
+
+// Raw (untyped, stride 0) buffer resource over [base, base + bytes), bytes
+// clamped to [0, 2^31 - 1]: a raw buffer load at a byte offset at or past the
+// range returns zeros (dword 3 of a gfx9 buffer descriptor: kRawBufferFlags).
+constexpr int kRawBufferFlags = 0x00020000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base, int64_t bytes) {
+    const int n = bytes <= 0 ? 0 : bytes >= 0x7fffffff ? 0x7fffffff : (int)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n,
+                                             kRawBufferFlags);
+}
 constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, one L2 each
 
 // Row, inside a 16x16 result block, of accumulator element v of lane 16 g + r
