@@ -402,53 +402,59 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g) {
 // result element j of lane l is C[8 (j / 4) + 4 h + j % 4][c].
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-__global__ __launch_bounds__(512) void bf16x3_nt_kernel(Bf16x3Gemm g) {
+template <int WN>
+__global__ __launch_bounds__(128 * WN) void bf16x3_nt_kernel(Bf16x3Gemm g) {
     constexpr int T = 256, KP = 32;
+    constexpr int NT = 128 * WN;        // threads: 2 waves along M x WN along N
+    constexpr int WNE = T / WN;         // N extent of one wave
+    constexpr int NY = WNE / 32;        // 32-wide blocks along N per wave
+    constexpr int RP = NT / 4;          // tile rows one load instruction covers
+    constexpr int LPO = 2 * (T / RP);   // loads per thread per operand (hi and lo)
     __shared__ __attribute__((aligned(16))) v4i lds[2][2][T][8];
     const int mt = (g.M + T - 1) / T, nt = (g.N + T - 1) / T;
     const int64_t tile = xcd_remap((int64_t)blockIdx.x, (int64_t)mt * nt);
     const int m0 = (int)(tile % mt) * T, n0 = (int)(tile / mt) * T;
     const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-    const int wm = wid / 4, wn = wid % 4, c = lane % 32, h = lane / 32;
+    const int wm = wid / WN, wn = wid % WN, c = lane % 32, h = lane / 32;
 
-    // chunk q = tid + 512 u: operand u >> 2 (A for u < 4, B after), row
-    // (tid >> 3) + 64 (u & 3), piece tid & 7 (pieces 0-3 hi, 4-7 lo, each 8
-    // consecutive k). Row pointers are formed once; rows past M or N read row
-    // 0 of their operand and are zeroed.
-    const int piece = tid & 7;
-    const bool hi_piece = piece < 4;
-    const uint16_t *const a_base = hi_piece ? g.ahi : g.alo;
-    const uint16_t *const b_base = hi_piece ? g.bhi : g.blo;
-    const uint16_t *src[8];
-    bool ok[8];
+    // Load u of this thread: operand u / LPO (A, then B), hi (u even) or lo
+    // (u odd), row (tid >> 2) + RP ((u % LPO) >> 1) of the tile, piece
+    // (u & 1) * 4 + (tid & 3) of its 64-byte LDS row (8 consecutive k each).
+    // Every load instruction reads one array through a raw buffer resource
+    // over the tile's rows that exist, so rows past M or N read as zeros
+    // without a branch; voff is the byte offset of the row's piece.
+    const int crow = tid >> 2, cpiece = tid & 3;
+    const __amdgpu_buffer_rsrc_t rs[4] = {
+        raw_rsrc(g.ahi + (int64_t)m0 * g.lda, (int64_t)std::min(T, g.M - m0) * g.lda * 2),
+        raw_rsrc(g.alo + (int64_t)m0 * g.lda, (int64_t)std::min(T, g.M - m0) * g.lda * 2),
+        raw_rsrc(g.bhi + (int64_t)n0 * g.ldb, (int64_t)std::min(T, g.N - n0) * g.ldb * 2),
+        raw_rsrc(g.blo + (int64_t)n0 * g.ldb, (int64_t)std::min(T, g.N - n0) * g.ldb * 2)};
+    int voff[2 * LPO];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int row = (tid >> 3) + 64 * (u & 3);
-        const int grow = (u < 4 ? m0 : n0) + row;
-        ok[u] = grow < (u < 4 ? g.M : g.N);
-        const int64_t r = ok[u] ? grow : 0;
-        src[u] = (u < 4 ? a_base + r * g.lda : b_base + r * g.ldb) + (piece & 3) * 8;
+    for (int u = 0; u < 2 * LPO; ++u) {
+        const int row = crow + RP * ((u % LPO) >> 1);
+        voff[u] = (int)(((int64_t)row * (u < LPO ? g.lda : g.ldb) + cpiece * 8) * 2);
     }
-    v4i stg[8];
+    v4i stg[2 * LPO];
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const v4i v = *(const v4i *)(src[u] + k0);
-            stg[u] = ok[u] ? v : (v4i){0, 0, 0, 0};
-        }
+        for (int u = 0; u < 2 * LPO; ++u)
+            stg[u] = __builtin_bit_cast(
+                v4i, __builtin_amdgcn_raw_buffer_load_b128(rs[(u / LPO) * 2 + (u & 1)],
+                                                           voff[u] + 2 * k0, 0, 0));
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int row = (tid >> 3) + 64 * (u & 3);
-            lds[buf][u >> 2][row][piece ^ ((row >> 1) & 7)] = stg[u];
+        for (int u = 0; u < 2 * LPO; ++u) {
+            const int row = crow + RP * ((u % LPO) >> 1), piece = (u & 1) * 4 + cpiece;
+            lds[buf][u / LPO][row][piece ^ ((row >> 1) & 7)] = stg[u];
         }
     };
-    v16f acc[4][2];
+    v16f acc[4][NY];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+        for (int y = 0; y < NY; ++y)
 #pragma unroll
             for (int j = 0; j < 16; ++j) acc[x][y][j] = 0.0f;
     auto frag = [&](int buf, int op, int row, int chunk) -> v8bf {
@@ -457,7 +463,7 @@ __global__ __launch_bounds__(512) void bf16x3_nt_kernel(Bf16x3Gemm g) {
     auto compute = [&](int buf) {
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
-            v8bf ha[4], la[4], hb[2], lb[2];
+            v8bf ha[4], la[4], hb[NY], lb[NY];
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
                 const int ra = wm * 128 + x * 32 + c;
@@ -465,15 +471,15 @@ __global__ __launch_bounds__(512) void bf16x3_nt_kernel(Bf16x3Gemm g) {
                 la[x] = frag(buf, 0, ra, 4 + 2 * kk + h);
             }
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                const int rb = wn * 64 + y * 32 + c;
+            for (int y = 0; y < NY; ++y) {
+                const int rb = wn * WNE + y * 32 + c;
                 hb[y] = frag(buf, 1, rb, 2 * kk + h);
                 lb[y] = frag(buf, 1, rb, 4 + 2 * kk + h);
             }
 #pragma unroll
             for (int x = 0; x < 4; ++x)
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
+                for (int y = 0; y < NY; ++y) {
                     v16f a = acc[x][y];
                     a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la[x], hb[y], a, 0, 0, 0);
                     a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha[x], lb[y], a, 0, 0, 0);
@@ -498,8 +504,8 @@ __global__ __launch_bounds__(512) void bf16x3_nt_kernel(Bf16x3Gemm g) {
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            const int n = n0 + wn * 64 + y * 32 + c;
+        for (int y = 0; y < NY; ++y) {
+            const int n = n0 + wn * WNE + y * 32 + c;
             if (n >= g.N) continue;
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
@@ -711,9 +717,11 @@ void launch_fill(T *p, int64_t n, T v, hipStream_t st) {
 
 hipError_t launch_bf16x3_nt(const Bf16x3Gemm &g, hipStream_t st) {
     if (g.M <= 0 || g.N <= 0 || g.K < 0 || g.K % 32 != 0) return hipErrorInvalidValue;
+    // Byte offsets inside a 256-row tile are 32-bit (raw buffer loads).
+    if ((int64_t)256 * std::max(g.lda, g.ldb) * 2 >= 0x7fffffffLL) return hipErrorInvalidValue;
     const int64_t tiles = (int64_t)((g.M + 255) / 256) * ((g.N + 255) / 256);
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bf16x3_nt_kernel, dim3((unsigned)tiles), dim3(512), 0, st, g);
+    hipLaunchKernelGGL(bf16x3_nt_kernel<4>, dim3((unsigned)tiles), dim3(512), 0, st, g);
     return hipGetLastError();
 }
 
