@@ -231,11 +231,11 @@ fh_status check_kappa(double kappa, bool *has) {
 }
 
 // Built-in tile side (the tuning table overrides it per precision and size).
-// Large virtual spaces: 64 for FP64 / FP32 / FP8, whose 64-wide kernels run two
+// Large virtual spaces: 64 for FP64 / FP32, whose 64-wide kernels run two
 // workgroups (two waves per SIMD) per CU and keep the matrix pipes busier than
 // the one-workgroup 128-wide kernels (FP64 at 100/900/3600: 669 vs 729 ms);
-// 128 for BF16x3, whose 128-wide tile halves the operand traffic per MFMA
-// (193 vs 204 ms).
+// 128 for BF16x3 and FP8, whose eight-wave 128-wide kernels run two waves per
+// SIMD (133 vs 199 ms and 43 vs 58 ms against the 64-wide ones).
 int choose_tile(size_t nvir, Prec prec) {
     if (nvir <= 24) return 32;
     if (nvir <= 80 || (prec != Prec::BF16X3 && prec != Prec::FP8)) return 64;
