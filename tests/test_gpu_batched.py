@@ -119,6 +119,59 @@ def test_mixed_tile_sides_in_one_launch(small_set, small_dev, oracle_mod, prec):
             assert abs(e_os[d] - want[0]) <= bnd[0] and abs(e_ss[d] - want[1]) <= bnd[1], d
 
 
+def _edge_set():
+    """Domains at the edges of the tiling: one occupied and one virtual
+    orbital, one auxiliary function, virtual counts just past a 16- and a
+    64-multiple, auxiliary counts below and just past a panel, domains laid out
+    with gaps between their blocks."""
+    rng = np.random.default_rng(2024)
+    shapes = [(1, 1, 1), (1, 17, 7), (2, 16, 8), (3, 65, 9), (5, 33, 17), (4, 64, 129),
+              (12, 80, 3), (1, 129, 40), (7, 20, 100)]
+    nocc = np.array([x[0] for x in shapes])
+    nvir = np.array([x[1] for x in shapes])
+    naux = np.array([x[2] for x in shapes])
+    gap = 5
+    sizes = naux * nocc * nvir
+    b_off = np.concatenate([[0], np.cumsum(sizes + gap)[:-1]]).astype(np.int64)
+    e_off = np.concatenate([[0], np.cumsum(nocc + nvir + gap)[:-1]]).astype(np.int64)
+    b = np.full(int(b_off[-1] + sizes[-1] + gap), np.nan)
+    eps = np.full(int(e_off[-1] + nocc[-1] + nvir[-1] + gap), np.nan)
+    for d in range(len(shapes)):
+        no, nv, na = shapes[d]
+        b[b_off[d]:b_off[d] + na * no * nv] = synthetic._b(rng, na, no, nv).ravel()
+        eps[e_off[d]:e_off[d] + no + nv] = np.concatenate(
+            [np.sort(rng.uniform(-1.2, -0.35, no)), np.sort(rng.uniform(0.12, 5.0, nv))])
+    return synthetic.DomainSet(2024, nocc, nvir, naux, b_off, e_off, b, eps)
+
+
+@pytest.mark.parametrize("tile", ["", "32", "64"])
+def test_edge_shapes_against_oracle(oracle_mod, tile, monkeypatch):
+    """NaN-filled gaps between the blocks would poison any read outside a
+    domain's own block."""
+    if tile:
+        monkeypatch.setenv("FERRIC_HIP_TILE", tile)
+    ds = _edge_set()
+    bd = torch.from_numpy(ds.b_flat).cuda()
+    ed = torch.from_numpy(ds.eps_flat).cuda()
+    for prec in (Precision.FP64, Precision.BF16X3):
+        print("batched edge shapes", tile or "table", prec.name, flush=True)
+        e_os, e_ss = domain_set_energies(ds, prec, use_graph=False, b_dev=bd, eps_dev=ed)
+        for d in range(len(ds)):
+            p = ds.domain(d)
+            if prec == Precision.FP64:
+                want = oracle_mod.spin_components(p.b_ov, p.eps, p.nocc, p.nvir)
+                tol = 1e-12 * max(1.0, abs(want[2]))
+                assert abs(e_os[d] - want[0]) <= tol and abs(e_ss[d] - want[1]) <= tol, d
+            else:
+                hi, lo = quant.bf16_split(p.b_ov)
+                g = hi.T @ hi + hi.T @ lo + lo.T @ hi + np.diag(np.sum(lo * lo, axis=0))
+                want = oracle_mod.spin_components_from_g(g, p.eps, p.nocc, p.nvir)
+                bnd = oracle_mod.energy_bound(hi + lo, p.eps, p.nocc, p.nvir,
+                                              eps_g=(p.naux + 32) * U32)
+                assert abs(e_os[d] - want[0]) <= bnd[0] + 1e-15, d
+                assert abs(e_ss[d] - want[1]) <= bnd[1] + 1e-15, d
+
+
 def test_full_512_domains_against_single_problem_path():
     ds = synthetic.batched_domains(512)
     b = torch.from_numpy(ds.b_flat).cuda()
