@@ -5,11 +5,13 @@ every candidate tile side on a synthetic set of local-MP2 pair domains drawn
 from that bucket (nocc_d uniform on {2..12}; nvir_d uniform and naux_d
 log-uniform inside the bucket) through the batched path, and keeps the
 fastest. The table lives in ferric_amd/tuning/tiles.json and is installed into
-the library (fh_tune_set) whenever the library loads; the library takes the
-first entry whose bounds hold the problem (nvir <= nvir_max and naux <=
-naux_max, entries in table order), and problems outside every bucket keep the
-built-in choice. The last auxiliary bucket of each virtual bucket is open
-upwards.
+the library (fh_tune_set) whenever the library loads; for each batched domain
+the library takes the first entry whose bounds hold it (nvir <= nvir_max and
+naux <= naux_max, entries in table order), and domains outside every bucket
+keep the built-in choice. The last auxiliary bucket of each virtual bucket is
+open upwards. One-problem calls do not consult the table: their kernels (and
+the 128-wide eight-wave ones among them) are chosen by the library's measured
+built-in rule.
 
     python -m ferric_amd.autotune [--out PATH]   # measure on the current GPU, write the table
 """

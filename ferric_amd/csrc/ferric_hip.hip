@@ -453,9 +453,13 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
 
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec) {
     EnergyPlan p;
+    // The tile table is measured on the batched kernel and applies to batched
+    // domains; one-problem calls take the built-in choice unless
+    // FERRIC_HIP_TILE forces a tile.
+    (void)naux;
     p.tile = choose_tile(nvir, prec);
-    const int tuned = tuned_tile(prec, nvir, naux);
-    if (tuned > 0) p.tile = tuned;
+    const int forced = forced_tile();
+    if (forced > 0) p.tile = forced;
     if (prec == Prec::I8X4 && p.tile > 64) p.tile = 64;  // 64 and 32 are built
     p.nvir_pad = (int)((nvir + 15) / 16 * 16);
     const int kp = std::max(panel_depth(prec), prec == Prec::FP8 ? 128 : 0);

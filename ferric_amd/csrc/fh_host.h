@@ -112,8 +112,11 @@ fh_status check_kappa(double kappa, bool *has);
 fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t depth,
                       hipStream_t stream);
 EnergyPlan plan_energy(size_t nocc, size_t nvir, size_t naux, Prec prec);
-int tuned_tile(Prec prec, size_t nvir, size_t naux);
-// tuned_tile of n problems at once (one environment read, one lock).
+// FERRIC_HIP_TILE's tile side, or 0 when it is unset.
+int forced_tile();
+// Tile side of n batched domains: FERRIC_HIP_TILE's if set, else the first
+// tile-table entry whose bounds hold the domain, else 0 (one environment
+// read, one lock).
 void tuned_tiles(Prec prec, size_t n, const size_t *nvir, const size_t *naux, int *out);
 size_t packed_elem_bytes(Prec p);
 // The device's rocBLAS handle (created on first use, atomics off so that its

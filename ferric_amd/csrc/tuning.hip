@@ -1,8 +1,10 @@
 // Tile selection for the pair-energy kernels.
 //
-// A tile table (written by ferric_amd/autotune.py and installed through
-// fh_tune_set) maps (precision, nvir bucket, naux bucket) to the measured best
-// tile side; FERRIC_HIP_TILE forces one tile for every call.
+// A tile table (written by ferric_amd/autotune.py, which times the batched
+// domain kernel, and installed through fh_tune_set) maps (precision, nvir
+// bucket, naux bucket) to the measured best tile side of a batched domain;
+// one-problem calls keep their built-in choice. FERRIC_HIP_TILE forces one
+// tile for every call of either kind.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -23,7 +25,7 @@ std::mutex g_tune_mu;
 std::vector<TuneEntry> g_table;
 }  // namespace
 
-static int forced_tile() {
+int forced_tile() {
     const char *env = getenv("FERRIC_HIP_TILE");
     if (env && *env) {
         const int t = atoi(env);
@@ -37,12 +39,6 @@ static int table_tile(Prec prec, size_t nvir, size_t naux) {
     for (const auto &e : g_table)
         if (e.prec == (int)prec && nvir <= e.nvir_max && naux <= e.naux_max) return e.tile;
     return 0;
-}
-
-int tuned_tile(Prec prec, size_t nvir, size_t naux) {
-    if (const int t = forced_tile()) return t;
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    return table_tile(prec, nvir, naux);
 }
 
 void tuned_tiles(Prec prec, size_t n, const size_t *nvir, const size_t *naux, int *out) {

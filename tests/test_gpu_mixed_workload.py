@@ -99,3 +99,38 @@ def test_int8x4_exact_format_on_hot_rows(hot_small, oracle_mod):
     print(f"  dE vs FP64 {sc.e_os - f64[0]:+.2e} / {sc.e_ss - f64[1]:+.2e} (bound {bos:.2e} / "
           f"{bss:.2e})", flush=True)
     assert abs(sc.e_os - f64[0]) <= bos and abs(sc.e_ss - f64[1]) <= bss
+
+
+@pytest.mark.parametrize("prec", [Precision.BF16X3, Precision.FP8_E4M3])
+def test_eight_wave_kernels_odd_panels_thin_tile(oracle_mod, prec):
+    """The eight-wave BF16x3 / FP8 kernels (tile 128 past 80 virtuals): 150
+    virtuals leave a 32-row last tile, and 330 auxiliary functions give an odd
+    panel count (11 BF16 panels of 32, 3 FP8 panels of 128), so the waves of
+    a single-product item that take the odd panels meet one past naux_pad."""
+    from ferric_amd._lib import last_timing, set_timing
+
+    p = synthetic.mixed_large(seed=1017, nocc=6, nvir=150, naux=330)
+    b, eps, nocc, nvir = p.b_ov.astype(np.float64), p.eps, p.nocc, p.nvir
+    print("eight-wave", prec.name, flush=True)
+    set_timing(True)
+    try:
+        sc = rimp2.spin_components(torch.from_numpy(p.b_ov).cuda(), eps, nocc, nvir,
+                                   precision=prec)
+        assert last_timing()["tile"] == 128
+    finally:
+        set_timing(False)
+    eps_g = (p.naux + 32) * U32
+    if prec == Precision.FP8_E4M3:
+        bq = quant.fp8_dequantized(b, nocc, nvir)
+        want = oracle_mod.spin_components(bq, eps, nocc, nvir)
+        bnd = oracle_mod.energy_bound(bq, eps, nocc, nvir, eps_g=eps_g)
+    else:
+        hi, lo = quant.bf16_split(b)
+        g = hi.T @ hi + hi.T @ lo + lo.T @ hi
+        g += np.diag(np.sum(lo * lo, axis=0))
+        want = oracle_mod.spin_components_from_g(g, eps, nocc, nvir)
+        bnd = oracle_mod.energy_bound(hi + lo, eps, nocc, nvir, eps_g=eps_g)
+    print(f"  dE vs exact format {sc.e_os - want[0]:+.2e} / {sc.e_ss - want[1]:+.2e} "
+          f"(bound {bnd[0]:.2e} / {bnd[1]:.2e})", flush=True)
+    assert abs(sc.e_os - want[0]) <= bnd[0]
+    assert abs(sc.e_ss - want[1]) <= bnd[1]
