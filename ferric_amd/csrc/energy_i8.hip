@@ -350,21 +350,26 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
         for (int x = 0; x < NB; ++x)
 #pragma unroll
             for (int y = 0; y < NB; ++y) {
-                if (!(rv[x] && cv[y])) continue;
+                if (!(rv[x] && cv[y])) continue;  // wave-uniform
                 const int cb = L.wc * TW + y * 16 + L.r;
-                if (B0 + cb >= a.nvir) continue;
+                const bool okb = B0 + cb < a.nvir;
                 const double eb = eps_t[1][cb];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
+                    // Branch-free: a virtual past nvir adds an exact 0.0, which
+                    // leaves both compensated sums bit for bit as they were
+                    // (eps_t holds 0.0 there, so d stays eij: finite, nonzero).
                     const int ra = L.wr * TW + x * 16 + out_row<float>(L.g, v);
-                    if (A0 + ra >= a.nvir) continue;
+                    const bool ok = okb && A0 + ra < a.nvir;
                     const double d = eij - eps_t[0][ra] - eb;
                     const double g1v = g1[x][y][v], g2v = g2[x][y][v];
                     const double inv = 1.0 / d;
                     const double dd = g1v - g2v;
                     const double dmp = pair::damping(kap, a.kappa, d);
-                    os.add(dmp * ((g1v * g1v + g2v * g2v) * inv));
-                    ss.add(dmp * ((dd * dd) * inv));
+                    const double to = dmp * ((g1v * g1v + g2v * g2v) * inv);
+                    const double ts = dmp * ((dd * dd) * inv);
+                    os.add(ok ? to : 0.0);
+                    ss.add(ok ? ts : 0.0);
                 }
             }
     }

@@ -121,17 +121,20 @@ __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item 
             ea[v] = a.eps_vir[min(L.A0 + L.wr * TW + x * 16 + row_of(L.g, v), last)];
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
-            if (!(rv[x] && cv[y])) continue;
+            if (!(rv[x] && cv[y])) continue;  // wave-uniform
             const int bb = L.B0 + L.wc * TW + y * 16 + L.r;
-            if (bb >= a.nvir) continue;
+            const bool okb = bb < a.nvir;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
+                // Branch-free: a virtual past nvir adds an exact 0.0, which
+                // leaves both compensated sums bit for bit as they were (its
+                // clamped eps keeps d finite and nonzero).
                 const int aa = L.A0 + L.wr * TW + x * 16 + row_of(L.g, v);
-                if (aa >= a.nvir) continue;
+                const bool ok = okb && aa < a.nvir;
                 const double d = eij - ea[v] - eb[y];
                 double g1 = acc1[x][y][v], g2 = acc2[x][y][v];
                 if constexpr (kCorr) {
-                    if (it.i == it.j && aa == bb) {
+                    if (ok && it.i == it.j && aa == bb) {
                         const double c = corr(it.i, aa);
                         g1 += c;
                         g2 += c;
@@ -140,8 +143,10 @@ __device__ __forceinline__ void energy_epilogue(const EnergyArgs &a, const Item 
                 const double inv = 1.0 / d;
                 const double dd = g1 - g2;
                 const double dmp = damping(kap, a.kappa, d);
-                os.add(dmp * ((g1 * g1 + g2 * g2) * inv));
-                ss.add(dmp * ((dd * dd) * inv));
+                const double to = dmp * ((g1 * g1 + g2 * g2) * inv);
+                const double ts = dmp * ((dd * dd) * inv);
+                os.add(ok ? to : 0.0);
+                ss.add(ok ? ts : 0.0);
             }
         }
     }
