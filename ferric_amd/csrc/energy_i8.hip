@@ -39,53 +39,6 @@ namespace fh {
 namespace {
 
 constexpr int kLevels = 5;  // digit-product levels L = 2..6, index L - 2
-#ifndef FH_I8_BAND
-#define FH_I8_BAND 4
-#endif
-constexpr int kBand = FH_I8_BAND;  // tile rows per band of the tile-pair order
-
-// Workgroup b runs on XCD b % 8. Item of workgroup b: with the occupied pairs
-// in groups of eight, XCD x takes pair x of every group (pairs x, x + 8, ...)
-// and walks each pair's tile pairs in order, so at any moment the eight XCDs
-// work on eight neighbouring pairs, which mostly share their i rows: the
-// blocks one XCD misses in its L2 are often in the shared MALL from another.
-// Pairs past the last whole group keep the plain order. Bijective on
-// [0, nitems).
-__device__ __forceinline__ int64_t pair_interleave(int64_t b, int64_t ntp, int64_t npairs) {
-    const int64_t head = npairs / kXcds * kXcds * ntp;
-    if (b >= head) return b;
-    const int64_t x = b % kXcds, l = b / kXcds;
-    return ((l / ntp) * kXcds + x) * ntp + l % ntp;
-}
-
-// Tile pair t over nt tiles (tA <= tB), this kernel's order: the off-diagonal
-// pairs in bands of kBand consecutive tA rows, tB ascending within a band and
-// tA ascending within each tB; then the nt diagonal pairs. Consecutive items
-// share the B-side blocks of one tB and revisit the band's few A-side blocks
-// every kBand items, so both stay in L2 between their uses (row-major order
-// revisits a B-side block only one whole tA row later). Bijective on
-// [0, nt (nt + 1) / 2).
-__device__ __forceinline__ void decode_tile_pair_banded(int t, int nt, int &ta, int &tb) {
-    const int noff = nt * (nt - 1) / 2;
-    ta = tb = 0;
-    if (t >= noff) {
-        ta = tb = t - noff;
-        return;
-    }
-    for (int r0 = 0; r0 < nt; r0 += kBand) {
-        const int r1 = min(r0 + kBand, nt);
-        for (int b = r0 + 1; b < nt; ++b) {
-            const int n = min(b, r1) - r0;  // rows of the band above the diagonal in column b
-            if (t < n) {
-                ta = r0 + t;
-                tb = b;
-                return;
-            }
-            t -= n;
-        }
-    }
-}
-
 // sum_L 2^(8 (8 - L)) S_L * 2^(e - 60), summed from the smallest level up.
 __device__ __forceinline__ double combine_levels(int s2, int s3, int s4, int s5, int s6, int e) {
     if (e >= kI8NonFiniteSum) return NAN;  // either row non-finite

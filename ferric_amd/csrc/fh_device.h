@@ -103,6 +103,53 @@ __device__ __forceinline__ void decode_tile_pair(int t, int nt, int &ta, int &tb
     tb = a + 1 + rem;
 }
 
+#ifndef FH_TILE_BAND
+#define FH_TILE_BAND 4
+#endif
+constexpr int kTileBand = FH_TILE_BAND;  // tile rows per band of the banded tile-pair order
+
+// Workgroup b runs on XCD b % 8. Item of workgroup b for items numbered pair
+// by pair (ntp tile pairs each): with the occupied pairs in groups of eight,
+// XCD x takes pair x of every group (pairs x, x + 8, ...) and walks each
+// pair's tile pairs in order, so at any moment the eight XCDs work on eight
+// neighbouring pairs, which mostly share their i rows: the blocks one XCD
+// misses in its L2 are often in the shared MALL from another. Pairs past the
+// last whole group keep the plain order. Bijective on [0, npairs * ntp).
+__device__ __forceinline__ int64_t pair_interleave(int64_t b, int64_t ntp, int64_t npairs) {
+    const int64_t head = npairs / kXcds * kXcds * ntp;
+    if (b >= head) return b;
+    const int64_t x = b % kXcds, l = b / kXcds;
+    return ((l / ntp) * kXcds + x) * ntp + l % ntp;
+}
+
+// Tile pair t over nt tiles (tA <= tB), banded order: the off-diagonal pairs
+// in bands of kTileBand consecutive tA rows, tB ascending within a band and tA
+// ascending within each tB; then the nt diagonal pairs. Consecutive items
+// share the B-side blocks of one tB and revisit the band's few A-side blocks
+// every kTileBand items, so both stay in L2 between their uses (row-major
+// order revisits a B-side block only one whole tA row later). Bijective on
+// [0, nt (nt + 1) / 2).
+__device__ __forceinline__ void decode_tile_pair_banded(int t, int nt, int &ta, int &tb) {
+    const int noff = nt * (nt - 1) / 2;
+    ta = tb = 0;
+    if (t >= noff) {
+        ta = tb = t - noff;
+        return;
+    }
+    for (int r0 = 0; r0 < nt; r0 += kTileBand) {
+        const int r1 = min(r0 + kTileBand, nt);
+        for (int b = r0 + 1; b < nt; ++b) {
+            const int n = min(b, r1) - r0;  // rows of the band above the diagonal in column b
+            if (t < n) {
+                ta = r0 + t;
+                tb = b;
+                return;
+            }
+            t -= n;
+        }
+    }
+}
+
 // Inverse of decode_tile_pair.
 __device__ __forceinline__ int encode_tile_pair(int ta, int tb, int nt) {
     if (ta == tb) return nt * (nt - 1) / 2 + ta;
