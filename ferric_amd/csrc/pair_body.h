@@ -205,11 +205,12 @@ __device__ __forceinline__ void tstore(void *buf, bool f32, int64_t idx, double 
 // holding the tile pass active = true; further waves of a larger workgroup
 // (the INT8 kernel's) pass false and only join the barriers and the block sum.
 // FULLT: T~ goes to the row group's slabs at (i a, j b) and (i b, j a), with
-// their mirrors, instead of the row blocks tbuf / ubuf. A lane's elements run
-// along b, so the two of the four writes whose slab rows are indexed by b
-// (T~_ij^ab's mirror and T~_ij^ba) first go through LDS (tlds, two TILE x
-// (TILE + 1) words) and are written after a barrier by all threads of the
-// workgroup running along a: every slab store is then a contiguous run.
+// their mirrors, instead of the row blocks tbuf / ubuf. The tile's T~_ij^ab
+// and T~_ij^ba go through LDS (tlds, two TILE x (TILE + 1) words) and all
+// four writes are made after a barrier by all threads of the workgroup, along
+// b for the slab rows indexed by a and along a for those indexed by b: every
+// store instruction then writes whole 64-byte runs (a lane's own elements run
+// along b in 16-lane pieces, which left half-filled runs).
 template <int TILE, typename Acc, typename RowOfT, typename Corr, bool FULLT = false>
 __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item &it,
                                                  const Lane &L,
@@ -266,16 +267,10 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
                 }
                 if constexpr (FULLT) {
                     constexpr int LT = TILE + 1;
-                    const uint32_t h1 = bf16_pair(T1);
                     const int la = L.wr * TW + x * 16 + row_of(L.g, v);
                     const int lb = L.wc * TW + y * 16 + L.r;
-                    slab_put(a, it.i, aa, it.j, bb, h1);
-                    tlds[la * LT + lb] = h1;
-                    if (!diag_tile) {
-                        const uint32_t h2 = bf16_pair(T2);
-                        slab_put_mirror(a, it.i, bb, it.j, aa, h2);
-                        tlds[TILE * LT + la * LT + lb] = h2;
-                    }
+                    tlds[la * LT + lb] = bf16_pair(T1);
+                    if (!diag_tile) tlds[TILE * LT + la * LT + lb] = bf16_pair(T2);
                 } else {
                     // T~_ij^ab and T~_ij^ba into the row-i blocks.
                     tstore(a.tbuf, a.t_f32,
@@ -333,9 +328,18 @@ __device__ __forceinline__ void adjoint_epilogue(const EnergyArgs &a, const Item
         vec[2 * TILE] = s;
     }
     if constexpr (FULLT) {
-        // The transposed half of the slab writes (tlds was filled before the
-        // barrier above): thread runs along a, the slab rows are indexed by b.
+        // The slab writes, from tlds (filled before the barrier above) by all
+        // threads of the workgroup: first the slab rows indexed by a (threads
+        // run along b), then those indexed by b (threads run along a), so every
+        // store instruction writes whole 64-byte runs of a slab row.
         constexpr int LT = TILE + 1;
+        for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
+            const int la = e / TILE, lb = e % TILE;
+            const int aa = L.A0 + la, bb = L.B0 + lb;
+            if (aa >= a.nvir_pad || bb >= a.nvir_pad) continue;
+            slab_put(a, it.i, aa, it.j, bb, tlds[la * LT + lb]);
+            if (!diag_tile) slab_put_mirror(a, it.i, bb, it.j, aa, tlds[TILE * LT + la * LT + lb]);
+        }
         for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
             const int la = e % TILE, lb = e / TILE;
             const int aa = L.A0 + la, bb = L.B0 + lb;
