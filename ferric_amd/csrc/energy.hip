@@ -40,14 +40,9 @@ namespace fh {
 
 namespace {
 
-// Item of this workgroup: occupied pairs interleaved over the XCDs; the tile
-// pairs of an energy-only launch in the banded order (BANDED), those of the
-// adjoint and amplitude launches in decode_tile_pair's order, which their row
-// reductions and stores assume.
-template <bool BANDED>
 __device__ __forceinline__ pair::Item decode_item(const EnergyArgs &a) {
     pair::Item it;
-    it.id = pair_interleave((int64_t)blockIdx.x, a.ntp, a.nitems / a.ntp);
+    it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
     const int64_t p = it.id / a.ntp;
     const int tp = (int)(it.id % a.ntp);
     if (a.i_fixed >= 0) {
@@ -56,10 +51,7 @@ __device__ __forceinline__ pair::Item decode_item(const EnergyArgs &a) {
     } else {
         decode_pair(p, a.nocc, it.i, it.j);
     }
-    if (BANDED)
-        decode_tile_pair_banded(tp, a.ntile, it.ta, it.tb);
-    else
-        decode_tile_pair(tp, a.ntile, it.ta, it.tb);
+    decode_tile_pair(tp, a.ntile, it.ta, it.tb);
     return it;
 }
 
@@ -67,8 +59,7 @@ template <typename T, int TILE, int KP, bool ADJ, bool SIGNED = false, bool FLUS
           bool AMP = false>
 __global__ __launch_bounds__(256) void pair_k4_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::K4Lds<T, TILE, KP> lds;
-    pair::pair_k4_body<T, TILE, KP, ADJ, false, SIGNED, FLUSH, AMP>(
-        a, decode_item<!ADJ && !AMP>(a), lds);
+    pair::pair_k4_body<T, TILE, KP, ADJ, false, SIGNED, FLUSH, AMP>(a, decode_item(a), lds);
 }
 
 // Opposite-spin rectangle: item (i, J, tA, tB), J and tB on the beta side;
@@ -93,13 +84,13 @@ __global__ __launch_bounds__(256) void pair_rect_kernel(EnergyArgs a) {
 template <int TILE>
 __global__ __launch_bounds__(512) void pair_bf16x3_w8_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::Bf16Lds<TILE> lds;
-    pair::pair_bf16x3_w8_body<TILE>(a, decode_item<true>(a), lds);
+    pair::pair_bf16x3_w8_body<TILE>(a, decode_item(a), lds);
 }
 
 template <int TILE, bool ADJ>
 __global__ __launch_bounds__(256) void pair_bf16x3_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::Bf16Lds<TILE> lds;
-    pair::pair_bf16x3_body<TILE, ADJ>(a, decode_item<!ADJ>(a), lds);
+    pair::pair_bf16x3_body<TILE, ADJ>(a, decode_item(a), lds);
 }
 
 // ---------------------------------------------------------------------------
