@@ -318,27 +318,42 @@ __global__ __launch_bounds__(256) void pack_i8x4_kernel(const Tin *__restrict__ 
         tile[q][tx] = v;
     }
     __syncthreads();
-    for (int rr = ty; rr < PT; rr += 4) {
-        const int64_t orow = r0 + rr;
-        if (orow >= nrows) continue;
+    // Digit phase: thread t takes packed row r0 + rr, rr = 16 (t / 64) + t % 16,
+    // and the 16 Q values 16 g .. 16 g + 15 of the panel, g = (t / 16) % 4:
+    // the 16 bytes of lane 16 g + rr % 16 in each digit's kilobyte of the MFMA
+    // fragment order (block (row group orow / 16, panel q0 / 64) of 4 KB, digit
+    // by digit), so every wave writes four whole kilobytes.
+    const int t = threadIdx.x;
+    const int g = (t / 16) % 4, rr = (t / 64) * 16 + t % 16;
+    const int64_t orow = r0 + rr;
+    if (orow < nrows) {
         const int e = rexp[orow];
-        int y = 0;
-        if (e != kI8NonFinite) y = (int)rint(ldexp(tile[tx][rr], 30 - e));
-        const int d4 = (int)(int8_t)(y & 0xff);
-        y = (y - d4) >> 8;
-        const int d3 = (int)(int8_t)(y & 0xff);
-        y = (y - d3) >> 8;
-        const int d2 = (int)(int8_t)(y & 0xff);
-        const int d1 = (y - d2) >> 8;
-        // MFMA fragment order: block (row group orow / 16, panel q0 / 64) of
-        // 4 KB, digit by digit; lane 16 g + r (r = orow % 16, g = tx / 16)
-        // holds Q = q0 + 16 g .. q0 + 16 g + 15 at 16 (16 g + r).
+        uint32_t w[4][4];  // [digit d1..d4][word]: byte k % 4 of word k / 4 is Q = 16 g + k
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[d][k] = 0u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            int y = 0;
+            if (e != kI8NonFinite) y = (int)rint(ldexp(tile[16 * g + k][rr], 30 - e));
+            const int d4 = (int)(int8_t)(y & 0xff);
+            y = (y - d4) >> 8;
+            const int d3 = (int)(int8_t)(y & 0xff);
+            y = (y - d3) >> 8;
+            const int d2 = (int)(int8_t)(y & 0xff);
+            const int d1 = (y - d2) >> 8;
+            const int sh = 8 * (k % 4);
+            w[0][k / 4] |= (uint32_t)(uint8_t)d1 << sh;
+            w[1][k / 4] |= (uint32_t)(uint8_t)d2 << sh;
+            w[2][k / 4] |= (uint32_t)(uint8_t)d3 << sh;
+            w[3][k / 4] |= (uint32_t)(uint8_t)d4 << sh;
+        }
         int8_t *o = out + ((orow / 16) * (int64_t)(naux_pad / 64) + q0 / 64) * 4096 +
-                    ((tx / 16) * 16 + (int)(orow % 16)) * 16 + tx % 16;
-        o[0] = (int8_t)d1;
-        o[1024] = (int8_t)d2;
-        o[2048] = (int8_t)d3;
-        o[3072] = (int8_t)d4;
+                    (g * 16 + (int)(orow % 16)) * 16;
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+            *(uint4 *)(o + 1024 * d) = make_uint4(w[d][0], w[d][1], w[d][2], w[d][3]);
     }
 }
 
