@@ -79,12 +79,13 @@ __global__ __launch_bounds__(256) void pair_rect_kernel(EnergyArgs a) {
     pair::pair_k4_body<T, TILE, KP, false, true, false, FLUSH>(a, it, lds);
 }
 
-// BF16x3 energy on eight waves (pair_bf16x3_w8_body): two waves per SIMD at
-// TILE = 128.
+
+// BF16x3 energy on eight waves from the fragment-order operand
+// (pair_bf16x3_w8f_body): two waves per SIMD at TILE = 128.
 template <int TILE>
-__global__ __launch_bounds__(512) void pair_bf16x3_w8_kernel(EnergyArgs a) {
+__global__ __launch_bounds__(512) void pair_bf16x3_w8f_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::Bf16Lds<TILE> lds;
-    pair::pair_bf16x3_w8_body<TILE>(a, decode_item(a), lds);
+    pair::pair_bf16x3_w8f_body<TILE>(a, decode_item(a), lds);
 }
 
 template <int TILE, bool ADJ>
@@ -186,7 +187,9 @@ hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
         else if (a.tile == 32) e = launch_items(pair_k4_kernel<float, 32, 16, false, false, true>, a, st);
         break;
     case Prec::BF16X3:
-        if (a.tile == 128) e = launch_items(pair_bf16x3_w8_kernel<128>, a, st, 512);
+        if (a.tile == 128) {
+            if (a.opf) e = launch_items(pair_bf16x3_w8f_kernel<128>, a, st, 512);
+        }
         else if (a.tile == 64) e = launch_items(pair_bf16x3_kernel<64, false>, a, st);
         else if (a.tile == 32) e = launch_items(pair_bf16x3_kernel<32, false>, a, st);
         break;

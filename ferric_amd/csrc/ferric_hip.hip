@@ -319,8 +319,12 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
         {kSlotResult, sizeof(double) * (2 + 2 * nocc)},
         {kSlotScalar, 256},
     };
+    // BF16x3 at tile 128: hi and lo in MFMA fragment order in one buffer, read
+    // by the eight-wave kernel straight into registers.
+    const bool frag = prec == Prec::BF16X3 && pl.tile == 128 && !signed_k && !amp_out;
+    if (frag) req[0].second = 2 * op_bytes;
     if (prec == Prec::BF16X3) {
-        req.push_back({kSlotOp1, op_bytes});
+        if (!frag) req.push_back({kSlotOp1, op_bytes});
         req.push_back({kSlotDiag, sizeof(double) * rows});
     }
     if (prec == Prec::FP8) req.push_back({kSlotScale, nocc * nqb});
@@ -367,11 +371,11 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     pa.naux_pad = pl.naux_pad;
     pa.fmt = prec == Prec::F64      ? PackFmt::F64
              : prec == Prec::F32 || prec == Prec::MIXED ? PackFmt::F32
-             : prec == Prec::BF16X3 ? PackFmt::BF16X2
+             : prec == Prec::BF16X3 ? (frag ? PackFmt::BF16F : PackFmt::BF16X2)
              : prec == Prec::I8X4   ? PackFmt::I8X4
                                     : PackFmt::FP8;
     pa.out0 = s.slots[kSlotOp0].ptr;
-    pa.out1 = prec == Prec::BF16X3 ? s.slots[kSlotOp1].ptr : nullptr;
+    pa.out1 = prec == Prec::BF16X3 && !frag ? s.slots[kSlotOp1].ptr : nullptr;
     pa.scale = prec == Prec::FP8 ? (uint8_t *)s.slots[kSlotScale].ptr : nullptr;
     pa.lo_sq = prec == Prec::BF16X3 ? (double *)s.slots[kSlotDiag].ptr : nullptr;
     pa.rexp = prec == Prec::I8X4 ? (int *)s.slots[kSlotScale].ptr : nullptr;
@@ -404,6 +408,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     ea.scale = pa.scale;
     ea.diag_corr = pa.lo_sq;
     ea.rexp = pa.rexp;
+    ea.opf = frag ? pa.out0 : nullptr;
     ea.nocc = (int)nocc;
     ea.nvir = (int)nvir;
     ea.nvir_pad = pl.nvir_pad;

@@ -357,6 +357,80 @@ __global__ __launch_bounds__(256) void pack_i8x4_kernel(const Tin *__restrict__ 
     }
 }
 
+// BF16 hi / lo split (ConvBf16Split) written in MFMA fragment order
+// (PackFmt::BF16F): per (16-row group, 32-deep Q panel) one 2 KB block, the hi
+// fragment then the lo fragment, lane l = 16 g + r at byte 16 l holding row r,
+// Q = 8 g .. 8 g + 7 of the panel. One 64 x 64 tile per block (as
+// transpose_tile); thread t then writes lane t % 64 of row group t / 64 for
+// both panels of the tile, so every wave writes whole kilobytes.
+template <typename Tin>
+__global__ __launch_bounds__(256) void pack_bf16f_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                         int naux, int nvir, int nvir_pad,
+                                                         int64_t nrows, int naux_pad,
+                                                         char *__restrict__ out) {
+    __shared__ float tile[PT][PT + 1];
+    const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;
+    const int64_t r0 = (int64_t)blockIdx.x * PT;
+    const int q0 = blockIdx.y * PT;
+    const int64_t r = r0 + tx;
+    int64_t col = -1;
+    if (r < nrows) {
+        const int64_t i = r / nvir_pad;
+        const int a = (int)(r % nvir_pad);
+        if (a < nvir) col = i * nvir + a;
+    }
+    for (int q = ty; q < PT; q += 4) {
+        const int qq = q0 + q;
+        float v = 0.0f;
+        if (col >= 0 && qq < naux) v = load_as_float(b + (int64_t)qq * nov + col);
+        tile[q][tx] = v;
+    }
+    __syncthreads();
+    const int l = threadIdx.x % 64, g = l / 16, rr = (threadIdx.x / 64) * 16 + l % 16;
+    const int64_t orow = r0 + rr;
+    if (orow >= nrows) return;
+    const int np = naux_pad / 32;
+#pragma unroll
+    for (int pp = 0; pp < 2; ++pp) {
+        const int P = q0 / 32 + pp;
+        if (P >= np) break;
+        uint32_t hw[4], lw[4];
+#pragma unroll
+        for (int k = 0; k < 8; k += 2) {
+            uint16_t h0, l0, h1, l1;
+            ConvBf16Split{}(tile[32 * pp + 8 * g + k][rr], &h0, &l0);
+            ConvBf16Split{}(tile[32 * pp + 8 * g + k + 1][rr], &h1, &l1);
+            hw[k / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            lw[k / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+        }
+        char *o = out + ((orow / 16) * (int64_t)np + P) * 2048 + l * 16;
+        *(uint4 *)o = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        *(uint4 *)(o + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    }
+}
+
+// lo_sq_kernel over the fragment-order layout: the same values summed in the
+// same order (lane q sums Q = q, q + 64, ..., then the xor tree).
+__global__ __launch_bounds__(256) void lo_sq_frag_kernel(const char *__restrict__ frag,
+                                                         int64_t nrows, int naux_pad,
+                                                         double *__restrict__ out) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= nrows) return;
+    const int np = naux_pad / 32, r = (int)(row % 16);
+    const char *base = frag + (row / 16) * (int64_t)np * 2048 + 1024 + r * 16;
+    double s = 0.0;
+    for (int q = lane; q < naux_pad; q += kWave) {
+        const int P = q / 32, g = (q % 32) / 8, e = q % 8;
+        const uint16_t u = *(const uint16_t *)(base + (int64_t)P * 2048 + g * 256 + e * 2);
+        const float v = (float)__builtin_bit_cast(__bf16, u);
+        s += (double)(v * v);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (lane == 0) out[row] = s;
+}
+
 }  // namespace
 
 static inline dim3 pack_grid(int64_t nrows, int naux_pad) {
@@ -392,6 +466,13 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t st) {
         if (a.lo_sq)
             hipLaunchKernelGGL(lo_sq_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, st,
                                (const uint16_t *)a.out1, nrows, a.naux_pad, a.lo_sq);
+        break;
+    case PackFmt::BF16F:
+        if (a.naux_pad % 32 != 0 || nrows % 16 != 0) return hipErrorInvalidValue;
+        FH_LAUNCH(pack_bf16f_kernel, (char *)a.out0);
+        if (a.lo_sq)
+            hipLaunchKernelGGL(lo_sq_frag_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0,
+                               st, (const char *)a.out0, nrows, a.naux_pad, a.lo_sq);
         break;
     case PackFmt::FP8: {
         const int nqb = a.naux_pad / 64;

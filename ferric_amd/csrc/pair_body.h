@@ -883,89 +883,53 @@ __device__ __forceinline__ void pair_bf16x3_body(const EnergyArgs &a, const Item
         energy_epilogue<TILE>(a, it, L, acc1, acc2, rv, cv, RowOf<float>{}, corr);
 }
 
-// The same BF16 hi/lo three-pass contraction on eight waves (512 threads):
-// waves 0-3 accumulate C1 (operands 0 and 1), waves 4-7 C2 (operands 2 and 3)
-// or, for a single-product item, C1 over the odd Q panels; each wave one
-// TILE/2-square quadrant. Two waves share each SIMD (against one for the
-// four-wave body at TILE = 128), so one wave's LDS and barrier waits overlap
-// the other's MFMAs; the LDS bytes read per MFMA are those of the four-wave
-// body. After the loop the halves meet through LDS (FP32 C1 + odd-panel C1 for
-// a single-product item, in the four-wave body's order, so the results are
+// The same BF16 hi/lo three-pass contraction on eight waves (512 threads),
+// every fragment loaded straight into registers from the operand in MFMA
+// fragment order (EnergyArgs::opf, PackFmt::BF16F): per (16-row group, 32-deep
+// Q panel) one 2 KB block, hi then lo, each a coalesced 1 KB wave load, so the
+// main loop has no LDS traffic and no barrier (the INT8X4 kernel's scheme).
+// Waves 0-3 accumulate C1 (operand rows (i, A) and (j, B)), waves 4-7 C2 ((j, A)
+// and (i, B)) or, for a single-product item, C1 over the odd Q panels; each
+// wave one TILE/2-square quadrant, two waves per SIMD, one panel in flight
+// (double-buffered registers). Blocks past nvir_pad are neither loaded nor
+// multiplied. After the loop the halves meet through the idle LDS (the
+// odd-panel C1 added to C1 in the four-wave body's order, so the sums are
 // bitwise those of pair_bf16x3_body), and waves 0-3 run the energy epilogue
 // while waves 4-7 contribute zeros to the workgroup sum.
 template <int TILE>
-__device__ __forceinline__ void pair_bf16x3_w8_body(const EnergyArgs &a, const Item &it,
-                                                    Bf16Lds<TILE> &lds) {
+__device__ __forceinline__ void pair_bf16x3_w8f_body(const EnergyArgs &a, const Item &it,
+                                                     Bf16Lds<TILE> &lds) {
     constexpr int KP = 32;
     constexpr int TW = TILE / 2;
     constexpr int NB = TW / 16;
-    constexpr int CPR = 8;               // 4 hi + 4 lo chunks per row
-    constexpr int OPCH = TILE * CPR;
-    constexpr int NCH = 4 * OPCH;
-    constexpr int CPT = (NCH + 511) / 512;
+    constexpr int FBLK = 2048;  // bytes of one (group, panel) block: hi and lo fragments
     static_assert(sizeof(lds) >= sizeof(float) * TILE * (TILE + 1),
                   "LDS holds a whole FP32 tile after the loop");
-
-    const uint16_t *__restrict__ hi = (const uint16_t *)a.op0;
-    const uint16_t *__restrict__ lo = (const uint16_t *)a.op1;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
     const SplitQ sq(it, a.naux_pad, KP);
-    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
-    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
-    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
-    const int64_t ld = a.naux_pad;
-    const int tid = threadIdx.x;
-
-    // Chunk t of this thread: operand t / 2, hi (t even) or lo (t odd), row
-    // tid / 4, 16-byte piece tid % 4 of that half (8 consecutive Q), so every
-    // load instruction reads one array. The loads are raw buffer loads over
-    // each operand's valid rows (up to the last tile's nvir_pad), so rows past
-    // it read as zeros without a branch; a Q panel past naux_pad (the odd
-    // panel of a single-product item) gets an empty range. voff: the chunk's
-    // byte offset in its operand's rows, fixed for the item; LDS slots
-    // likewise.
-    static_assert(TILE * 4 == 512 && CPT == 8, "one chunk per thread per operand half");
-    const int64_t obase[4] = {base0 * ld, base1 * ld, base2 * ld, base3 * ld};
-    const int64_t obytes[4] = {(int64_t)limA * ld * 2, (int64_t)limB * ld * 2,
-                               (int64_t)limA * ld * 2, (int64_t)limB * ld * 2};
-    const int crow = tid >> 2, cpiece = tid & 3;
-    const int voff = (int)(((int64_t)crow * ld + cpiece * 8) * 2);
-    int slot[CPT];
-#pragma unroll
-    for (int t = 0; t < CPT; ++t) {
-        const int op = t >> 1, piece = (t & 1) * 4 + cpiece;
-        slot[t] = (op * TILE + crow) * CPR + (piece ^ (crow & 7));
-    }
-    v4i stg[CPT];
-    auto gload = [&](int q0) {
-        const int q23 = q0 + sq.off23;
-#pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            const int op = t >> 1;
-            const int q = op < 2 ? q0 : q23;
-            const __amdgpu_buffer_rsrc_t rs =
-                raw_rsrc(((t & 1) ? lo : hi) + obase[op], q < a.naux_pad ? obytes[op] : 0);
-            stg[t] = __builtin_bit_cast(
-                v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 2 * q, 0, 0));
-        }
-    };
-    auto lstore = [&](int buf) {
-        v4i *const dst = &lds[buf][0][0][0];
-#pragma unroll
-        for (int t = 0; t < CPT; ++t) dst[slot[t]] = stg[t];
-    };
-
-    const int lane = tid % kWave, wid = tid / kWave;
+    const int np = a.naux_pad / KP, ng = a.nvir_pad / 16;
+    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
     const int prod = wid >> 2, quad = wid & 3;
     const Lane L{A0, B0, quad >> 1, quad & 1, lane % 16, lane / 16};
-    const int opA = prod == 0 ? 0 : 2, opB = prod == 0 ? 1 : 3;
+    const bool c2 = prod == 1 && !sq.single;
+    const int occA = c2 ? it.j : it.i, occB = c2 ? it.i : it.j;
+    const int pstep = sq.single ? 2 : 1, poff = (prod == 1 && sq.single) ? 1 : 0;
+    const char *fbase = (const char *)a.opf;
+    const char *pa[NB], *pb[NB];
     bool rv[NB], cv[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
         cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
+        pa[k] = fbase + (int64_t)(occA * ng + (A0 + L.wr * TW) / 16 + k) * np * FBLK + lane * 16;
+        pb[k] = fbase + (int64_t)(occB * ng + (B0 + L.wc * TW) / 16 + k) * np * FBLK + lane * 16;
+    }
+    bool full = true, anyr = false, anyc = false;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        full = full && rv[k] && cv[k];
+        anyr = anyr || rv[k];
+        anyc = anyc || cv[k];
     }
     v4f acc[NB][NB];
 #pragma unroll
@@ -973,42 +937,55 @@ __device__ __forceinline__ void pair_bf16x3_w8_body(const EnergyArgs &a, const I
 #pragma unroll
         for (int y = 0; y < NB; ++y) acc[x][y] = (v4f){0, 0, 0, 0};
 
-    auto frag = [&](int buf, int op, int row, int chunk) -> v8bf {
-        return __builtin_bit_cast(v8bf, lds[buf][op][row][chunk ^ (row & 7)]);
-    };
-    auto compute = [&](int buf) {
-        v8bf ha[NB], la[NB], hb[NB], lb[NB];
+    // [hi / lo][block]
+    v4i fa[2][2][NB], fb[2][2][NB];
+    auto fetch = [&](int P, v4i(&ra)[2][NB], v4i(&rb)[2][NB], auto fullc) {
+        const int64_t off = (int64_t)P * FBLK;
 #pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            const int ra = L.wr * TW + k * 16 + L.r, rb = L.wc * TW + k * 16 + L.r;
-            ha[k] = frag(buf, opA, ra, L.g);
-            la[k] = frag(buf, opA, ra, 4 + L.g);
-            hb[k] = frag(buf, opB, rb, L.g);
-            lb[k] = frag(buf, opB, rb, 4 + L.g);
-        }
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                if (decltype(fullc)::value || rv[k])
+                    ra[h][k] = *(const v4i *)(pa[k] + off + h * 1024);
+                if (decltype(fullc)::value || cv[k])
+                    rb[h][k] = *(const v4i *)(pb[k] + off + h * 1024);
+            }
+    };
+    auto mma = [&](const v4i(&ra)[2][NB], const v4i(&rb)[2][NB], auto fullc) {
 #pragma unroll
         for (int x = 0; x < NB; ++x)
 #pragma unroll
             for (int y = 0; y < NB; ++y)
-                if (rv[x] && cv[y]) {
+                if (decltype(fullc)::value || (rv[x] && cv[y])) {
+                    const v8bf ha = __builtin_bit_cast(v8bf, ra[0][x]);
+                    const v8bf la = __builtin_bit_cast(v8bf, ra[1][x]);
+                    const v8bf hb = __builtin_bit_cast(v8bf, rb[0][y]);
+                    const v8bf lb = __builtin_bit_cast(v8bf, rb[1][y]);
                     v4f c = acc[x][y];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la[x], hb[y], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ha[x], lb[y], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ha[x], hb[y], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la, hb, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ha, lb, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ha, hb, c, 0, 0, 0);
                     acc[x][y] = c;
                 }
     };
-
-    gload(0);
-    lstore(0);
+    // Iteration p covers panel p * pstep + poff; unrolled by two so that the
+    // register buffer of each step is static.
+    const int iters = sq.iters;
+    auto valid = [&](int p) { return p < iters && p * pstep + poff < np; };
+    auto run = [&](auto fullc) {
+        if (valid(0)) fetch(poff, fa[0], fb[0], fullc);
+        for (int p = 0; p < iters; p += 2) {
+            if (valid(p + 1)) fetch((p + 1) * pstep + poff, fa[1], fb[1], fullc);
+            if (valid(p)) mma(fa[0], fb[0], fullc);
+            if (valid(p + 2)) fetch((p + 2) * pstep + poff, fa[0], fb[0], fullc);
+            if (valid(p + 1)) mma(fa[1], fb[1], fullc);
+        }
+    };
+    if (full)
+        run(std::true_type{});
+    else if (anyr && anyc)
+        run(std::false_type{});
     __syncthreads();
-    for (int p = 0; p < sq.iters; ++p) {
-        const int cur = p & 1;
-        if (p + 1 < sq.iters) gload((p + 1) * sq.step);
-        compute(cur);
-        if (p + 1 < sq.iters) lstore(cur ^ 1);
-        __syncthreads();
-    }
 
     // Waves 4-7 hand their sums to waves 0-3 through LDS (a whole FP32 tile,
     // row-major with a one-element pad): the odd-panel C1 of a single-product
