@@ -14,6 +14,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fh_device.h"
 #include "fh_kernels.h"
 #include "pair_body.h"
@@ -158,143 +160,111 @@ __global__ __launch_bounds__(256) void energy_fp8_kernel(EnergyArgs a) {
                                 pair::NoCorr{});
 }
 
-// The same contraction on eight waves (512 threads) at TILE = 128: waves 0-3
-// accumulate C1 (operands 0 and 1), waves 4-7 C2 (operands 2 and 3) or, for a
-// single-product item, C1 over the odd panels; each wave one 64-square
-// quadrant, two waves per SIMD. Operand chunks arrive by raw buffer loads
-// over each operand's valid rows (zeros past the last tile's nvir_pad and for
-// a panel past naux_pad, without branches). After the loop the halves meet
-// through LDS in the four-wave kernel's order (so the sums are bitwise the
-// same), waves 0-3 run the energy epilogue and waves 4-7 add zeros to the
-// workgroup sum.
+// The same contraction on eight waves (512 threads) at TILE = 128, from the
+// operand in MFMA fragment order (EnergyArgs::opf, PackFmt::FP8F): per
+// (16-row group, 128-deep Q panel) one 2 KB block, lane 16 g + r at byte
+// 32 (16 g + r) holding the two 16-byte chunks g and g + 4 of row r's panel,
+// the 32 bytes energy_fp8_kernel's LDS fragment read returns. Waves 0-3
+// accumulate C1, waves 4-7 C2 or, for a single-product item, C1 over the odd
+// panels; each wave one 64-square quadrant, two waves per SIMD; each fragment
+// is one coalesced 2 KB wave load straight into registers (no LDS, no barrier
+// in the main loop), one panel in flight. Blocks past nvir_pad are neither
+// loaded nor multiplied. After the loop the halves meet through LDS in the
+// four-wave kernel's order (so the sums are bitwise the same), waves 0-3 run
+// the energy epilogue and waves 4-7 add zeros to the workgroup sum.
 template <int TILE>
-__global__ __launch_bounds__(512) void energy_fp8_w8_kernel(EnergyArgs a) {
+__global__ __launch_bounds__(512) void energy_fp8_w8f_kernel(EnergyArgs a) {
     constexpr int KP = 128;
     constexpr int TW = TILE / 2;
     constexpr int NB = TW / 16;
-    constexpr int CPR = 8;  // 16-byte chunks per 128-byte row
-    static_assert(TILE * 4 == 512, "one chunk per thread per operand half-row");
-
-    __shared__ __attribute__((aligned(16))) v4i lds[2][4][TILE][CPR];
-    static_assert(sizeof(lds) >= sizeof(float) * TILE * (TILE + 1),
-                  "LDS holds a whole FP32 tile after the loop");
+    constexpr int FBLK = 2048;
+    __shared__ __attribute__((aligned(16))) float xbuf[TILE * (TILE + 1)];
 
     pair::Item it;
     it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
     decode_pair(it.id / a.ntp, a.nocc, it.i, it.j);
     decode_tile_pair((int)(it.id % a.ntp), a.ntile, it.ta, it.tb);
 
-    const uint8_t *__restrict__ bp = (const uint8_t *)a.op0;
     const int A0 = it.ta * TILE, B0 = it.tb * TILE;
     const pair::SplitQ sq(it, a.naux_pad, KP);
-    const int64_t base0 = (int64_t)it.i * a.nvir_pad + A0;
-    const int64_t base1 = (int64_t)it.j * a.nvir_pad + B0;
-    const int64_t base2 = sq.single ? base0 : (int64_t)it.j * a.nvir_pad + A0;
-    const int64_t base3 = sq.single ? base1 : (int64_t)it.i * a.nvir_pad + B0;
-    const int limA = a.nvir_pad - A0, limB = a.nvir_pad - B0;
-    const int64_t ld = a.naux_pad;
-    const int tid = threadIdx.x;
-
-    // Chunk t of this thread: operand t / 2, row tid / 4, 16-byte piece
-    // (t % 2) * 4 + tid % 4 of the 128-byte panel row.
-    const int64_t obase[4] = {base0 * ld, base1 * ld, base2 * ld, base3 * ld};
-    const int64_t obytes[4] = {(int64_t)limA * ld, (int64_t)limB * ld, (int64_t)limA * ld,
-                               (int64_t)limB * ld};
-    const int crow = tid >> 2, cpiece = tid & 3;
-    int slot[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int op = t >> 1, piece = (t & 1) * 4 + cpiece;
-        slot[t] = (op * TILE + crow) * CPR + (piece ^ (crow & 7));
-    }
-    v4i stg[8];
-    auto gload = [&](int q0) {
-        const int q23 = q0 + sq.off23;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int op = t >> 1;
-            const int q = op < 2 ? q0 : q23;
-            const __amdgpu_buffer_rsrc_t rs =
-                raw_rsrc(bp + obase[op], q < a.naux_pad ? obytes[op] : 0);
-            const int voff = (int)((int64_t)crow * ld + q + ((t & 1) * 4 + cpiece) * 16);
-            stg[t] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
-        }
-    };
-    auto lstore = [&](int buf) {
-        v4i *const dst = &lds[buf][0][0][0];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) dst[slot[t]] = stg[t];
-    };
-
-    const int lane = tid % kWave, wid = tid / kWave;
+    const int np = a.naux_pad / KP, ng = a.nvir_pad / 16;
+    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
     const int prod = wid >> 2, quad = wid & 3;
     const pair::Lane L{A0, B0, quad >> 1, quad & 1, lane % 16, lane / 16};
     const int g = L.g;
-    const int opA = prod == 0 ? 0 : 2, opB = prod == 0 ? 1 : 3;
+    const bool c2 = prod == 1 && !sq.single;
+    const int occA = c2 ? it.j : it.i, occB = c2 ? it.i : it.j;
+    const int pstep = sq.single ? 2 : 1, poff = (prod == 1 && sq.single) ? 1 : 0;
+    const char *fbase = (const char *)a.opf;
+    const char *pa[NB], *pb[NB];
     bool rv[NB], cv[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         rv[k] = A0 + L.wr * TW + k * 16 < a.nvir_pad;
         cv[k] = B0 + L.wc * TW + k * 16 < a.nvir_pad;
+        pa[k] = fbase + (int64_t)(occA * ng + (A0 + L.wr * TW) / 16 + k) * np * FBLK + lane * 32;
+        pb[k] = fbase + (int64_t)(occB * ng + (B0 + L.wc * TW) / 16 + k) * np * FBLK + lane * 32;
     }
+    bool full = true, anyr = false, anyc = false;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        full = full && rv[k] && cv[k];
+        anyr = anyr || rv[k];
+        anyc = anyc || cv[k];
+    }
+    // Scales of panel P: the (i, 64-row Q block) bytes of this lane's rows,
+    // block 2 P + g / 2, as in energy_fp8_kernel.
+    const uint8_t *scA = a.scale + (int64_t)occA * a.nqb + g / 2;
+    const uint8_t *scB = a.scale + (int64_t)occB * a.nqb + g / 2;
     v4f acc[NB][NB];
 #pragma unroll
     for (int x = 0; x < NB; ++x)
 #pragma unroll
         for (int y = 0; y < NB; ++y) acc[x][y] = (v4f){0, 0, 0, 0};
 
-    const uint8_t *sc_i = a.scale + (int64_t)it.i * a.nqb;
-    const uint8_t *sc_j = a.scale + (int64_t)it.j * a.nqb;
-    auto frag = [&](int buf, int op, int row) -> v8i {
-        v4i lo = lds[buf][op][row][g ^ (row & 7)];
-        v4i hi = lds[buf][op][row][(g + 4) ^ (row & 7)];
-        return (v8i){lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-    };
-    // Scales as in energy_fp8_kernel: operands 0/1 at panel p01, operands 2/3
-    // with i and j exchanged or (single product) at panel p01 + 1.
-    auto compute = [&](int buf, int p) {
-        const int p01 = sq.single ? 2 * p : p;
-        const int qa = 2 * p01 + g / 2;
-        const int qn = min(qa + 2, a.nqb - 1);
-        int sa, sb;
-        if (prod == 0) {
-            sa = sc_i[qa];
-            sb = sc_j[qa];
-        } else {
-            sa = sq.single ? sc_i[qn] : sc_j[qa];
-            sb = sq.single ? sc_j[qn] : sc_i[qa];
-        }
-        v8i fa[NB], fb[NB];
+    v8i fa[2][NB], fb[2][NB];
+    int sa[2], sb[2];
+    auto fetch = [&](int P, v8i(&ra)[NB], v8i(&rb)[NB], int &s0, int &s1, auto fullc) {
+        const int64_t off = (int64_t)P * FBLK;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
-            fa[k] = frag(buf, opA, L.wr * TW + k * 16 + L.r);
-            fb[k] = frag(buf, opB, L.wc * TW + k * 16 + L.r);
+            if (decltype(fullc)::value || rv[k]) ra[k] = *(const v8i *)(pa[k] + off);
+            if (decltype(fullc)::value || cv[k]) rb[k] = *(const v8i *)(pb[k] + off);
         }
+        s0 = scA[2 * P];
+        s1 = scB[2 * P];
+    };
+    auto mma = [&](const v8i(&ra)[NB], const v8i(&rb)[NB], int s0, int s1, auto fullc) {
 #pragma unroll
         for (int x = 0; x < NB; ++x)
 #pragma unroll
             for (int y = 0; y < NB; ++y)
-                if (rv[x] && cv[y])
+                if (decltype(fullc)::value || (rv[x] && cv[y]))
                     acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                        fa[x], fb[y], acc[x][y], 0, 0, 0, sa, 0, sb);
+                        ra[x], rb[y], acc[x][y], 0, 0, 0, s0, 0, s1);
     };
-
-    gload(0);
-    lstore(0);
+    const int iters = sq.iters;
+    auto valid = [&](int p) { return p < iters && p * pstep + poff < np; };
+    auto run = [&](auto fullc) {
+        if (valid(0)) fetch(poff, fa[0], fb[0], sa[0], sb[0], fullc);
+        for (int p = 0; p < iters; p += 2) {
+            if (valid(p + 1)) fetch((p + 1) * pstep + poff, fa[1], fb[1], sa[1], sb[1], fullc);
+            if (valid(p)) mma(fa[0], fb[0], sa[0], sb[0], fullc);
+            if (valid(p + 2)) fetch((p + 2) * pstep + poff, fa[0], fb[0], sa[0], sb[0], fullc);
+            if (valid(p + 1)) mma(fa[1], fb[1], sa[1], sb[1], fullc);
+        }
+    };
+    if (full)
+        run(std::true_type{});
+    else if (anyr && anyc)
+        run(std::false_type{});
     __syncthreads();
-    for (int p = 0; p < sq.iters; ++p) {
-        const int cur = p & 1;
-        if (p + 1 < sq.iters) gload((p + 1) * sq.step);
-        compute(cur, p);
-        if (p + 1 < sq.iters) lstore(cur ^ 1);
-        __syncthreads();
-    }
 
     // Halves meet through LDS: odd-panel C1 added to C1 (single product), or
     // C2 read back; a diagonal tile pair's C2 is C1 transposed, i == j gives
     // C2 = C1.
     constexpr int LDT = TILE + 1;
-    float *xb = (float *)&lds[0][0][0][0];
+    float *xb = xbuf;
     auto put = [&](const v4f(&v)[NB][NB]) {
 #pragma unroll
         for (int x = 0; x < NB; ++x)
@@ -350,8 +320,8 @@ __global__ __launch_bounds__(512) void energy_fp8_w8_kernel(EnergyArgs a) {
 }  // namespace
 
 hipError_t launch_energy_fp8(const EnergyArgs &a, hipStream_t st) {
-    if (a.tile == 128)
-        hipLaunchKernelGGL(energy_fp8_w8_kernel<128>, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
+    if (a.tile == 128 && a.opf)
+        hipLaunchKernelGGL(energy_fp8_w8f_kernel<128>, dim3((unsigned)a.nitems), dim3(512), 0, st, a);
     else if (a.tile == 64)
         hipLaunchKernelGGL(energy_fp8_kernel<64>, dim3((unsigned)a.nitems), dim3(256), 0, st, a);
     else if (a.tile == 32)

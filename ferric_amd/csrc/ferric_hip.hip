@@ -319,10 +319,12 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
         {kSlotResult, sizeof(double) * (2 + 2 * nocc)},
         {kSlotScalar, 256},
     };
-    // BF16x3 at tile 128: hi and lo in MFMA fragment order in one buffer, read
-    // by the eight-wave kernel straight into registers.
-    const bool frag = prec == Prec::BF16X3 && pl.tile == 128 && !signed_k && !amp_out;
-    if (frag) req[0].second = 2 * op_bytes;
+    // BF16x3 and FP8 at tile 128: the operand (BF16: hi and lo in one buffer)
+    // in MFMA fragment order, read by the eight-wave kernels straight into
+    // registers.
+    const bool frag = (prec == Prec::BF16X3 || prec == Prec::FP8) && pl.tile == 128 && !signed_k &&
+                      !amp_out;
+    if (frag && prec == Prec::BF16X3) req[0].second = 2 * op_bytes;
     if (prec == Prec::BF16X3) {
         if (!frag) req.push_back({kSlotOp1, op_bytes});
         req.push_back({kSlotDiag, sizeof(double) * rows});
@@ -373,7 +375,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
              : prec == Prec::F32 || prec == Prec::MIXED ? PackFmt::F32
              : prec == Prec::BF16X3 ? (frag ? PackFmt::BF16F : PackFmt::BF16X2)
              : prec == Prec::I8X4   ? PackFmt::I8X4
-                                    : PackFmt::FP8;
+                                    : (frag ? PackFmt::FP8F : PackFmt::FP8);
     pa.out0 = s.slots[kSlotOp0].ptr;
     pa.out1 = prec == Prec::BF16X3 && !frag ? s.slots[kSlotOp1].ptr : nullptr;
     pa.scale = prec == Prec::FP8 ? (uint8_t *)s.slots[kSlotScale].ptr : nullptr;

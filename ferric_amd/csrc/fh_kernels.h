@@ -74,7 +74,10 @@ constexpr int kI8MaxNauxPad = 32832;
 // and 32-deep Q panel a 2 KB block: the hi then the lo fragment, lane 16 g + r
 // at byte 16 (16 g + r) holding row r, Q = 8 g .. 8 g + 7), read by the
 // eight-wave BF16x3 energy kernel straight into registers.
-enum class PackFmt { F64, F32, BF16X2, FP8, I8X4, BF16F };
+// FP8F: the FP8 bytes in MFMA fragment order (per 16-row group and 128-deep
+// Q panel a 2 KB block, lane 16 g + r at byte 32 (16 g + r) holding chunks g
+// and g + 4 of row r's 128-byte panel row), read by the eight-wave FP8 kernel.
+enum class PackFmt { F64, F32, BF16X2, FP8, I8X4, BF16F, FP8F };
 
 struct PackArgs {
     const void *b;    // caller layout, row-major (naux, nocc*nvir)
@@ -128,7 +131,7 @@ struct EnergyArgs {
     const uint8_t *scale;     // FP8 scales (nocc, nqb)
     const double *diag_corr;  // BF16: sum_Q lo^2 per packed row, added to K_ii[a, a]
     const int *rexp = nullptr;  // I8X4: per packed row, the exponent of its digit scale
-    const void *opf = nullptr;  // BF16X3, tile 128: hi / lo in MFMA fragment order (or null)
+    const void *opf = nullptr;  // BF16X3 / FP8, tile 128: the operand in MFMA fragment order
     int nocc, nvir, nvir_pad, naux_pad, nqb;
     const double *eps_occ;    // device, active occupied (nocc)
     const double *eps_vir;    // device, virtual (nvir)

@@ -199,6 +199,59 @@ __global__ __launch_bounds__(256) void pack_fp8_kernel(const Tin *__restrict__ b
     }
 }
 
+// The E4M3 bytes of pack_fp8_kernel in MFMA fragment order (PackFmt::FP8F):
+// per (16-row group, 128-deep Q panel) one 2 KB block, lane l = 16 g + r at
+// byte 32 l holding row r's Q = 16 g .. 16 g + 15 and then 64 + 16 g .. 64 + 16 g
+// + 15 of the panel (the two 16-byte chunks g and g + 4 of its 128-byte row).
+// One 128 Q x 64 row tile per block; thread t then writes lane t % 64 of row
+// group t / 64, so every wave writes two whole kilobytes.
+template <typename Tin>
+__global__ __launch_bounds__(256) void pack_fp8f_kernel(const Tin *__restrict__ b, int64_t nov,
+                                                        int naux, int nvir, int nvir_pad,
+                                                        int64_t nrows, int naux_pad, int nqb,
+                                                        const uint8_t *__restrict__ scale,
+                                                        char *__restrict__ out) {
+    __shared__ float tile[2 * PT][PT + 1];
+    const int tx = threadIdx.x % PT, ty = threadIdx.x / PT;
+    const int64_t r0 = (int64_t)blockIdx.x * PT;
+    const int P = blockIdx.y, q0 = P * 2 * PT;
+    const int64_t r = r0 + tx;
+    int64_t col = -1;
+    if (r < nrows) {
+        const int64_t i = r / nvir_pad;
+        const int a = (int)(r % nvir_pad);
+        if (a < nvir) col = i * nvir + a;
+    }
+    for (int q = ty; q < 2 * PT; q += 4) {
+        const int qq = q0 + q;
+        float v = 0.0f;
+        if (col >= 0 && qq < naux) v = (float)b[(int64_t)qq * nov + col];
+        tile[q][tx] = v;
+    }
+    __syncthreads();
+    const int l = threadIdx.x % 64, g = l / 16, rr = (threadIdx.x / 64) * 16 + l % 16;
+    const int64_t orow = r0 + rr;
+    if (orow >= nrows) return;
+    const int64_t i = orow / nvir_pad;
+    uint32_t w[8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e = (int)scale[i * nqb + 2 * P + h] - 127;
+#pragma unroll
+        for (int k = 0; k < 16; k += 4) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                word |= (uint32_t)f32_to_e4m3(ldexpf(tile[64 * h + 16 * g + k + u][rr], -e))
+                        << (8 * u);
+            w[4 * h + k / 4] = word;
+        }
+    }
+    char *o = out + ((orow / 16) * (int64_t)(naux_pad / (2 * PT)) + P) * 2048 + l * 32;
+    *(uint4 *)o = make_uint4(w[0], w[1], w[2], w[3]);
+    *(uint4 *)(o + 16) = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
 // Row sums of lo^2 over Q (FP64; each lo^2 is exact in FP32): the lo*lo term
 // the three-pass BF16 product omits. On the diagonal elements
 // K_ii[a, a] = sum_Q B^Q_ia^2 every omitted term is positive, so the omission
@@ -484,6 +537,20 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t st) {
             hipLaunchKernelGGL(fp8_scale_kernel<double>, sg, dim3(256), 0, st,
                                (const double *)a.b, nov, a.naux, a.nvir, nqb, a.scale);
         FH_LAUNCH(pack_fp8_kernel, nqb, (const uint8_t *)a.scale, (uint8_t *)a.out0);
+        break;
+    }
+    case PackFmt::FP8F: {
+        if (a.naux_pad % (2 * PT) != 0 || nrows % 16 != 0) return hipErrorInvalidValue;
+        const int nqb = a.naux_pad / 64;
+        dim3 sg((unsigned)nqb, (unsigned)a.nocc);
+        if (f32in)
+            hipLaunchKernelGGL(fp8_scale_kernel<float>, sg, dim3(256), 0, st, (const float *)a.b,
+                               nov, a.naux, a.nvir, nqb, a.scale);
+        else
+            hipLaunchKernelGGL(fp8_scale_kernel<double>, sg, dim3(256), 0, st,
+                               (const double *)a.b, nov, a.naux, a.nvir, nqb, a.scale);
+        grid.y = (unsigned)(a.naux_pad / (2 * PT));
+        FH_LAUNCH(pack_fp8f_kernel, nqb, (const uint8_t *)a.scale, (char *)a.out0);
         break;
     }
     case PackFmt::I8X4: {
