@@ -101,17 +101,13 @@ def test_int8x4_exact_format_on_hot_rows(hot_small, oracle_mod):
     assert abs(sc.e_os - f64[0]) <= bos and abs(sc.e_ss - f64[1]) <= bss
 
 
-@pytest.mark.parametrize("prec", [Precision.BF16X3, Precision.FP8_E4M3])
-def test_eight_wave_kernels_odd_panels_thin_tile(oracle_mod, prec):
-    """The eight-wave BF16x3 / FP8 kernels (tile 128 past 80 virtuals): 150
-    virtuals leave a 32-row last tile, and 330 auxiliary functions give an odd
-    panel count (11 BF16 panels of 32, 3 FP8 panels of 128), so the waves of
-    a single-product item that take the odd panels meet one past naux_pad."""
+def _eight_wave_check(oracle_mod, p, prec):
+    """Energy of one tile-128 BF16x3 / FP8 call against the FP64 oracle energy
+    of exactly the values the format contracts, within the FP32-accumulation
+    bound."""
     from ferric_amd._lib import last_timing, set_timing
 
-    p = synthetic.mixed_large(seed=1017, nocc=6, nvir=150, naux=330)
     b, eps, nocc, nvir = p.b_ov.astype(np.float64), p.eps, p.nocc, p.nvir
-    print("eight-wave", prec.name, flush=True)
     set_timing(True)
     try:
         sc = rimp2.spin_components(torch.from_numpy(p.b_ov).cuda(), eps, nocc, nvir,
@@ -134,3 +130,28 @@ def test_eight_wave_kernels_odd_panels_thin_tile(oracle_mod, prec):
           f"(bound {bnd[0]:.2e} / {bnd[1]:.2e})", flush=True)
     assert abs(sc.e_os - want[0]) <= bnd[0]
     assert abs(sc.e_ss - want[1]) <= bnd[1]
+
+
+@pytest.mark.parametrize("prec", [Precision.BF16X3, Precision.FP8_E4M3])
+def test_eight_wave_kernels_odd_panels_thin_tile(oracle_mod, prec):
+    """The eight-wave BF16x3 / FP8 kernels (tile 128 past 80 virtuals): 150
+    virtuals leave a 32-row last tile, and 330 auxiliary functions give an odd
+    panel count (11 BF16 panels of 32, 3 FP8 panels of 128), so the waves of
+    a single-product item that take the odd panels meet one past naux_pad."""
+    print("eight-wave", prec.name, flush=True)
+    _eight_wave_check(oracle_mod, synthetic.mixed_large(seed=1017, nocc=6, nvir=150, naux=330),
+                      prec)
+
+
+@pytest.mark.parametrize("prec", [Precision.BF16X3, Precision.FP8_E4M3])
+@pytest.mark.parametrize("shape", [(1, 97, 33), (2, 128, 32), (3, 257, 129), (5, 81, 200)])
+def test_eight_wave_fragment_order_shapes(oracle_mod, prec, shape):
+    """The fragment-order operands of the eight-wave kernels at awkward shapes:
+    one occupied orbital (every item single-product), a virtual count of
+    exactly one tile, one past two tiles (a 16-row last block group) and one
+    past 80, a single BF16 panel, auxiliary counts just past a panel."""
+    nocc, nvir, naux = shape
+    print("eight-wave fragment order", prec.name, shape, flush=True)
+    _eight_wave_check(oracle_mod,
+                      synthetic.mixed_large(seed=1100 + nocc, nocc=nocc, nvir=nvir, naux=naux),
+                      prec)
