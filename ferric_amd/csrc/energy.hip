@@ -81,7 +81,8 @@ __global__ __launch_bounds__(256) void pair_rect_kernel(EnergyArgs a) {
 
 
 // BF16x3 energy on eight waves from the fragment-order operand
-// (pair_bf16x3_w8f_body): two waves per SIMD at TILE = 128.
+// (pair_bf16x3_w8f_body), TILE = 128 or 64 (at 64: 106 VGPRs, two workgroups
+// per CU).
 template <int TILE>
 __global__ __launch_bounds__(512) void pair_bf16x3_w8f_kernel(EnergyArgs a) {
     __shared__ __attribute__((aligned(16))) pair::Bf16Lds<TILE> lds;
@@ -190,6 +191,7 @@ hipError_t launch_energy(const EnergyArgs &a, hipStream_t st) {
         if (a.tile == 128) {
             if (a.opf) e = launch_items(pair_bf16x3_w8f_kernel<128>, a, st, 512);
         }
+        else if (a.tile == 64 && a.opf) e = launch_items(pair_bf16x3_w8f_kernel<64>, a, st, 512);
         else if (a.tile == 64) e = launch_items(pair_bf16x3_kernel<64, false>, a, st);
         else if (a.tile == 32) e = launch_items(pair_bf16x3_kernel<32, false>, a, st);
         break;

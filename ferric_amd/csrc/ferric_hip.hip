@@ -230,16 +230,20 @@ fh_status check_kappa(double kappa, bool *has) {
     return FH_OK;
 }
 
-// Built-in tile side (the tuning table overrides it per precision and size).
+// Built-in tile side for one-problem calls (FERRIC_HIP_TILE forces one).
 // Large virtual spaces: 64 for FP64 / FP32, whose 64-wide kernels run two
 // workgroups (two waves per SIMD) per CU and keep the matrix pipes busier than
 // the one-workgroup 128-wide kernels (FP64 at 100/900/3600: 669 vs 729 ms);
-// 128 for BF16x3 and FP8, whose eight-wave 128-wide kernels run two waves per
-// SIMD (133 vs 199 ms and 43 vs 58 ms against the 64-wide ones).
+// 128 for FP8 past 80 virtuals and BF16x3 past 192, whose eight-wave 128-wide
+// kernels read fragment-order operands straight into registers (at
+// 100/900/3600 BF16x3 95 vs 216 ms, FP8 30 vs 58 ms against 64). Below 192
+// virtuals BF16x3's eight-wave kernel at 64 is faster (40/160/960: 0.28 vs
+// 0.34 ms; 41/209/960: 0.45 vs 0.42 ms).
 int choose_tile(size_t nvir, Prec prec) {
     if (nvir <= 24) return 32;
-    if (nvir <= 80 || (prec != Prec::BF16X3 && prec != Prec::FP8)) return 64;
-    return 128;  // the eight-wave BF16x3 and FP8 kernels
+    if (prec == Prec::BF16X3) return nvir <= 192 ? 64 : 128;
+    if (prec == Prec::FP8 && nvir > 80) return 128;
+    return 64;
 }
 
 size_t packed_elem_bytes(Prec p) {
@@ -322,8 +326,9 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     // BF16x3 and FP8 at tile 128: the operand (BF16: hi and lo in one buffer)
     // in MFMA fragment order, read by the eight-wave kernels straight into
     // registers.
-    const bool frag = (prec == Prec::BF16X3 || prec == Prec::FP8) && pl.tile == 128 && !signed_k &&
-                      !amp_out;
+    const bool frag = ((prec == Prec::BF16X3 && (pl.tile == 128 || pl.tile == 64)) ||
+                       (prec == Prec::FP8 && pl.tile == 128)) &&
+                      !signed_k && !amp_out;
     if (frag && prec == Prec::BF16X3) req[0].second = 2 * op_bytes;
     if (prec == Prec::BF16X3) {
         if (!frag) req.push_back({kSlotOp1, op_bytes});

@@ -101,10 +101,10 @@ def test_int8x4_exact_format_on_hot_rows(hot_small, oracle_mod):
     assert abs(sc.e_os - f64[0]) <= bos and abs(sc.e_ss - f64[1]) <= bss
 
 
-def _eight_wave_check(oracle_mod, p, prec):
-    """Energy of one tile-128 BF16x3 / FP8 call against the FP64 oracle energy
-    of exactly the values the format contracts, within the FP32-accumulation
-    bound."""
+def _eight_wave_check(oracle_mod, p, prec, tile=128):
+    """Energy of one BF16x3 / FP8 call at the given tile side against the FP64
+    oracle energy of exactly the values the format contracts, within the
+    FP32-accumulation bound."""
     from ferric_amd._lib import last_timing, set_timing
 
     b, eps, nocc, nvir = p.b_ov.astype(np.float64), p.eps, p.nocc, p.nvir
@@ -112,7 +112,7 @@ def _eight_wave_check(oracle_mod, p, prec):
     try:
         sc = rimp2.spin_components(torch.from_numpy(p.b_ov).cuda(), eps, nocc, nvir,
                                    precision=prec)
-        assert last_timing()["tile"] == 128
+        assert last_timing()["tile"] == tile
     finally:
         set_timing(False)
     eps_g = (p.naux + 32) * U32
@@ -133,11 +133,12 @@ def _eight_wave_check(oracle_mod, p, prec):
 
 
 @pytest.mark.parametrize("prec", [Precision.BF16X3, Precision.FP8_E4M3])
-def test_eight_wave_kernels_odd_panels_thin_tile(oracle_mod, prec):
-    """The eight-wave BF16x3 / FP8 kernels (tile 128 past 80 virtuals): 150
-    virtuals leave a 32-row last tile, and 330 auxiliary functions give an odd
-    panel count (11 BF16 panels of 32, 3 FP8 panels of 128), so the waves of
-    a single-product item that take the odd panels meet one past naux_pad."""
+def test_eight_wave_kernels_odd_panels_thin_tile(oracle_mod, prec, monkeypatch):
+    """The eight-wave BF16x3 / FP8 kernels at the 128-wide tile: 150 virtuals
+    leave a 32-row last tile, and 330 auxiliary functions give an odd panel
+    count (11 BF16 panels of 32, 3 FP8 panels of 128), so the waves of a
+    single-product item that take the odd panels meet one past naux_pad."""
+    monkeypatch.setenv("FERRIC_HIP_TILE", "128")
     print("eight-wave", prec.name, flush=True)
     _eight_wave_check(oracle_mod, synthetic.mixed_large(seed=1017, nocc=6, nvir=150, naux=330),
                       prec)
@@ -145,13 +146,17 @@ def test_eight_wave_kernels_odd_panels_thin_tile(oracle_mod, prec):
 
 @pytest.mark.parametrize("prec", [Precision.BF16X3, Precision.FP8_E4M3])
 @pytest.mark.parametrize("shape", [(1, 97, 33), (2, 128, 32), (3, 257, 129), (5, 81, 200)])
-def test_eight_wave_fragment_order_shapes(oracle_mod, prec, shape):
+@pytest.mark.parametrize("tile", [128, 64])
+def test_eight_wave_fragment_order_shapes(oracle_mod, prec, shape, tile, monkeypatch):
     """The fragment-order operands of the eight-wave kernels at awkward shapes:
     one occupied orbital (every item single-product), a virtual count of
     exactly one tile, one past two tiles (a 16-row last block group) and one
-    past 80, a single BF16 panel, auxiliary counts just past a panel."""
+    past 80, a single BF16 panel, auxiliary counts just past a panel; at the
+    128-wide and the 64-wide tile (the BF16x3 eight-wave kernel at both, FP8's
+    eight-wave kernel at 128 and its four-wave kernel at 64)."""
+    monkeypatch.setenv("FERRIC_HIP_TILE", str(tile))
     nocc, nvir, naux = shape
-    print("eight-wave fragment order", prec.name, shape, flush=True)
+    print("eight-wave fragment order", prec.name, shape, tile, flush=True)
     _eight_wave_check(oracle_mod,
                       synthetic.mixed_large(seed=1100 + nocc, nocc=nocc, nvir=nvir, naux=naux),
-                      prec)
+                      prec, tile)
