@@ -19,8 +19,9 @@
 //    tuning.hip, filled by ferric_amd/autotune.py); the kernel branches on it
 //    per item, over one shared LDS buffer.
 //  - FP64 reads the operand panels straight from the caller's Q-major blocks:
-//    no packing pass. BF16x3 packs hi / lo rows first (the split needs a pass
-//    of its own for the sum of lo^2 per row).
+//    no packing pass. BF16x3 packs hi / lo in MFMA fragment order first (the
+//    split needs a pass of its own for the sum of lo^2 per row) and runs the
+//    eight-wave fragment-order body (512 threads) at either tile side.
 //  - Thread 0 asks for its next item as soon as it starts one, so the queue
 //    atomic's latency hides behind the item's work.
 //
@@ -49,7 +50,7 @@ namespace {
 struct DomDev {
     int64_t b_off;      // caller B (row-major (naux, nocc*nvir)) offset, elements
     int64_t eps_off;    // [occupied..., virtual...] offset
-    int64_t bp_off;     // BF16: packed rows offset (elements), same in the lo arena
+    int64_t bp_off;     // BF16: Σ rows x naux_pad of the domains before (fragment order at 2 bp_off)
     int64_t row_off;    // BF16: first packed row (for the diagonal correction)
     int64_t item0;      // first work item of the domain
     int nocc, nvir, naux, nvir_pad, naux_pad, ntile, ntp, tile;
@@ -67,8 +68,8 @@ struct BatchArgs {
     const int *item_dom;      // domain of each work item
     const PackTile *ptiles;
     int64_t nptiles;
-    void *op0;                // BF16 hi
-    void *op1;                // BF16 lo
+    void *op0;                // BF16 hi / lo in MFMA fragment order (PackFmt::BF16F)
+    void *op1;                // unused
     double *corr;             // BF16 sum_Q lo^2 per packed row
     int64_t nrows;            // packed rows over all domains
     const int64_t *row_dom_start;  // first packed row of each domain (ndom + 1)
@@ -81,11 +82,15 @@ struct BatchArgs {
 
 // ---- packing (BF16x3) -------------------------------------------------------------
 // One 64 x 64 tile of one domain: rows r of the domain's packed block, Q in
-// [q0, q0 + 64), split into BF16 hi / lo. Same layout and zero padding as
-// pack.hip.
+// [q0, q0 + 64), split into BF16 hi / lo (the split of pack.hip) and written in
+// MFMA fragment order (PackFmt::BF16F): per (16-row group, 32-deep Q panel) a
+// 2 KB block, the hi then the lo fragment, lane l = 16 g + r at 16 l holding
+// row r, Q = 8 g .. 8 g + 7. The domain's blocks start at element 2 bp_off of
+// op0. Thread t writes lane t % 64 of row group t / 64 for both panels.
 __global__ __launch_bounds__(256) void batched_pack_kernel(BatchArgs a) {
     __shared__ double tile[64][65];
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int l = threadIdx.x % 64, g = l / 16, rr = (threadIdx.x / 64) * 16 + l % 16;
     for (int64_t t = blockIdx.x; t < a.nptiles; t += gridDim.x) {
         const PackTile pt = a.ptiles[t];
         const DomDev dm = a.dom[pt.d];
@@ -106,21 +111,35 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(BatchArgs a) {
             tile[q][tx] = v;
         }
         __syncthreads();
-        for (int rr = ty; rr < 64; rr += 4) {
-            const int64_t orow = pt.r0 + rr;
-            if (orow < nrows && pt.q0 + tx < dm.naux_pad) {
-                const int64_t o = dm.bp_off + orow * dm.naux_pad + pt.q0 + tx;
-                const float xf = (float)tile[tx][rr];
-                const __bf16 h = (__bf16)xf;
-                const __bf16 l = (__bf16)(xf - (float)h);
-                ((uint16_t *)a.op0)[o] = __builtin_bit_cast(uint16_t, h);
-                ((uint16_t *)a.op1)[o] = __builtin_bit_cast(uint16_t, l);
+        const int64_t orow = pt.r0 + rr;
+        if (orow >= nrows) continue;
+        const int np = dm.naux_pad / 32;
+        uint16_t *const frag = (uint16_t *)a.op0 + 2 * dm.bp_off;
+#pragma unroll
+        for (int pp = 0; pp < 2; ++pp) {
+            const int P = pt.q0 / 32 + pp;
+            if (P >= np) break;
+            uint32_t hw[4], lw[4];
+#pragma unroll
+            for (int k = 0; k < 8; k += 2) {
+                const float x0 = (float)tile[32 * pp + 8 * g + k][rr];
+                const float x1 = (float)tile[32 * pp + 8 * g + k + 1][rr];
+                const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
+                const __bf16 l0 = (__bf16)(x0 - (float)h0), l1 = (__bf16)(x1 - (float)h1);
+                hw[k / 2] = (uint32_t)__builtin_bit_cast(uint16_t, h0) |
+                            ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+                lw[k / 2] = (uint32_t)__builtin_bit_cast(uint16_t, l0) |
+                            ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
             }
+            uint16_t *o = frag + ((orow / 16) * (int64_t)np + P) * 1024 + l * 8;
+            *(uint4 *)o = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+            *(uint4 *)(o + 512) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
         }
     }
 }
 
-// BF16: sum over Q of lo^2 per packed row (one wave per row).
+// BF16: sum over Q of lo^2 per packed row (one wave per row), read from the
+// fragment order; lane q sums Q = q, q + 64, ... and the xor tree finishes.
 __global__ __launch_bounds__(256) void batched_lo_sq_kernel(BatchArgs a) {
     const int lane = threadIdx.x % kWave;
     for (int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave; row < a.nrows;
@@ -133,10 +152,13 @@ __global__ __launch_bounds__(256) void batched_lo_sq_kernel(BatchArgs a) {
         }
         const DomDev dm = a.dom[lo];
         const int64_t local = row - dm.row_off;
-        const uint16_t *p = (const uint16_t *)a.op1 + dm.bp_off + local * dm.naux_pad;
+        const int np = dm.naux_pad / 32;
+        const uint16_t *p = (const uint16_t *)a.op0 + 2 * dm.bp_off +
+                            (local / 16) * (int64_t)np * 1024 + 512 + (local % 16) * 8;
         double s = 0.0;
         for (int q = lane; q < dm.naux_pad; q += kWave) {
-            const float v = (float)__builtin_bit_cast(__bf16, p[q]);
+            const int P = q / 32, gq = (q % 32) / 8, e = q % 8;
+            const float v = (float)__builtin_bit_cast(__bf16, p[(int64_t)P * 1024 + gq * 128 + e]);
             s += (double)(v * v);
         }
 #pragma unroll
@@ -408,7 +430,9 @@ union BatchLdsBf {
     pair::Bf16Lds<32> t32;
 };
 
-__global__ __launch_bounds__(256) void batched_bf16_kernel(BatchArgs a) {
+// BF16x3 items on eight waves (512 threads) from the fragment-order operand
+// (pair_bf16x3_w8f_body at the item's tile side).
+__global__ __launch_bounds__(512) void batched_bf16_kernel(BatchArgs a) {
     __shared__ __attribute__((aligned(16))) BatchLdsBf lds;
     __shared__ int s_item;
     EnergyArgs e;
@@ -424,13 +448,12 @@ __global__ __launch_bounds__(256) void batched_bf16_kernel(BatchArgs a) {
         const DomDev dm = load_domain(a, item);
         pair::Item it;
         item_setup(a, dm, item, it, e);
-        e.op0 = (const uint16_t *)a.op0 + dm.bp_off;
-        e.op1 = (const uint16_t *)a.op1 + dm.bp_off;
+        e.opf = (const uint16_t *)a.op0 + 2 * dm.bp_off;
         e.diag_corr = a.corr + dm.row_off;
         if (dm.tile == 64)
-            pair::pair_bf16x3_body<64, false>(e, it, lds.t64);
+            pair::pair_bf16x3_w8f_body<64>(e, it, lds.t64);
         else
-            pair::pair_bf16x3_body<32, false>(e, it, lds.t32);
+            pair::pair_bf16x3_w8f_body<32>(e, it, lds.t32);
         if (threadIdx.x == 0) s_item = nxt;
         __syncthreads();
     }
@@ -661,7 +684,7 @@ static void enqueue(const BatchArgs &ba, const Plan &pl, Prec prec, hipStream_t 
     const int64_t grid = batch_grid(prec, pl.nitems);
     if (grid > 0) {
         if (prec == Prec::BF16X3)
-            hipLaunchKernelGGL(batched_bf16_kernel, dim3((unsigned)grid), dim3(256), 0, st, ba);
+            hipLaunchKernelGGL(batched_bf16_kernel, dim3((unsigned)grid), dim3(512), 0, st, ba);
         else
             hipLaunchKernelGGL(batched_fp64_kernel, dim3((unsigned)grid), dim3(256), 0, st, ba);
     }
@@ -725,8 +748,7 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
         {kSlotBatchOut, sizeof(double) * 2 * ndom + 64 * sizeof(int)},
     };
     if (bf) {
-        req.push_back({kSlotOp0, pl.packed * 2});
-        req.push_back({kSlotOp1, pl.packed * 2});
+        req.push_back({kSlotOp0, pl.packed * 4});  // hi and lo, 2 bytes each
         req.push_back({kSlotDiag, sizeof(double) * pl.nrows});
     }
     st = reserve(s, req);   // a slot that moves drops the graph cache (and the plan's upload)
@@ -779,7 +801,7 @@ extern "C" fh_status fh_rimp2_batched_dev(const fh_domain *domains, size_t ndom,
     ba.ptiles = (const PackTile *)(mbase + o_pt);
     ba.nptiles = (int64_t)pl.ptiles.size();
     ba.op0 = bf ? s.slots[kSlotOp0].ptr : nullptr;
-    ba.op1 = bf ? s.slots[kSlotOp1].ptr : nullptr;
+    ba.op1 = nullptr;
     ba.corr = bf ? (double *)s.slots[kSlotDiag].ptr : nullptr;
     ba.nrows = pl.nrows;
     ba.row_dom_start = (const int64_t *)(mbase + o_rs);
