@@ -138,32 +138,42 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(BatchArgs a) {
     }
 }
 
-// BF16: sum over Q of lo^2 per packed row (one wave per row), read from the
-// fragment order; lane q sums Q = q, q + 64, ... and the xor tree finishes.
+// BF16: sum over Q of lo^2 per packed row, one wave per group of 16 rows, read
+// from the fragment order: lane 16 g + r reads its 16-byte lo chunk (row r, Q =
+// 32 P + 8 g .. + 7) of every panel P, one contiguous kilobyte per wave load,
+// and sums its squares in Q order; the four lanes of a row then combine by
+// an xor pair (g, g ^ 1) and (g, g ^ 2). A domain's rows are whole groups
+// (nvir_pad is a multiple of 16).
 __global__ __launch_bounds__(256) void batched_lo_sq_kernel(BatchArgs a) {
     const int lane = threadIdx.x % kWave;
-    for (int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave; row < a.nrows;
-         row += (int64_t)gridDim.x * 4) {
-        // domain of this packed row: last d with row_dom_start[d] <= row
+    const int64_t ngroups = a.nrows / 16;
+    for (int64_t grp = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave; grp < ngroups;
+         grp += (int64_t)gridDim.x * 4) {
+        const int64_t row0 = grp * 16;
+        // domain of this group: last d with row_dom_start[d] <= row0
         int lo = 0, hi = a.ndom - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) / 2;
-            if (a.row_dom_start[mid] <= row) lo = mid; else hi = mid - 1;
+            if (a.row_dom_start[mid] <= row0) lo = mid; else hi = mid - 1;
         }
         const DomDev dm = a.dom[lo];
-        const int64_t local = row - dm.row_off;
+        const int64_t lg = (row0 - dm.row_off) / 16;
         const int np = dm.naux_pad / 32;
-        const uint16_t *p = (const uint16_t *)a.op0 + 2 * dm.bp_off +
-                            (local / 16) * (int64_t)np * 1024 + 512 + (local % 16) * 8;
+        const uint16_t *p = (const uint16_t *)a.op0 + 2 * dm.bp_off + lg * np * 1024 + 512 + lane * 8;
         double s = 0.0;
-        for (int q = lane; q < dm.naux_pad; q += kWave) {
-            const int P = q / 32, gq = (q % 32) / 8, e = q % 8;
-            const float v = (float)__builtin_bit_cast(__bf16, p[(int64_t)P * 1024 + gq * 128 + e]);
-            s += (double)(v * v);
-        }
+        for (int P = 0; P < np; ++P) {
+            const uint4 w = *(const uint4 *)(p + (int64_t)P * 1024);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
-        if (lane == 0) a.corr[row] = s;
+            for (int e = 0; e < 8; ++e) {
+                const float v = (float)__builtin_bit_cast(
+                    __bf16, (uint16_t)(ws[e / 2] >> (16 * (e % 2))));
+                s += (double)(v * v);
+            }
+        }
+        s += __shfl_xor(s, 16, kWave);
+        s += __shfl_xor(s, 32, kWave);
+        if (lane < 16) a.corr[row0 + lane] = s;
     }
 }
 
@@ -678,7 +688,7 @@ static void enqueue(const BatchArgs &ba, const Plan &pl, Prec prec, hipStream_t 
         const unsigned pblocks =
             (unsigned)std::min<int64_t>(std::max<int64_t>(pl.ptiles.size(), 1), 4096);
         hipLaunchKernelGGL(batched_pack_kernel, dim3(pblocks), dim3(256), 0, st, ba);
-        const unsigned rb = (unsigned)std::min<int64_t>((pl.nrows + 3) / 4, 8192);
+        const unsigned rb = (unsigned)std::min<int64_t>((pl.nrows / 16 + 3) / 4, 8192);
         hipLaunchKernelGGL(batched_lo_sq_kernel, dim3(std::max(rb, 1u)), dim3(256), 0, st, ba);
     }
     const int64_t grid = batch_grid(prec, pl.nitems);

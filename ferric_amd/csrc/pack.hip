@@ -462,26 +462,33 @@ __global__ __launch_bounds__(256) void pack_bf16f_kernel(const Tin *__restrict__
     }
 }
 
-// lo_sq_kernel over the fragment-order layout: the same values summed in the
-// same order (lane q sums Q = q, q + 64, ..., then the xor tree).
+// Sum over Q of lo^2 per packed row from the fragment-order layout, one wave
+// per group of 16 rows: lane 16 g + r reads its 16-byte lo chunk (row r, Q =
+// 32 P + 8 g .. + 7) of every panel P, one contiguous kilobyte per wave load,
+// and sums its squares in Q order; the four lanes of a row then combine by an
+// xor pair (g, g ^ 1) and (g, g ^ 2).
 __global__ __launch_bounds__(256) void lo_sq_frag_kernel(const char *__restrict__ frag,
                                                          int64_t nrows, int naux_pad,
                                                          double *__restrict__ out) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
+    const int64_t grp = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    if (row >= nrows) return;
-    const int np = naux_pad / 32, r = (int)(row % 16);
-    const char *base = frag + (row / 16) * (int64_t)np * 2048 + 1024 + r * 16;
+    if (grp * 16 >= nrows) return;
+    const int np = naux_pad / 32;
+    const char *p = frag + grp * (int64_t)np * 2048 + 1024 + lane * 16;
     double s = 0.0;
-    for (int q = lane; q < naux_pad; q += kWave) {
-        const int P = q / 32, g = (q % 32) / 8, e = q % 8;
-        const uint16_t u = *(const uint16_t *)(base + (int64_t)P * 2048 + g * 256 + e * 2);
-        const float v = (float)__builtin_bit_cast(__bf16, u);
-        s += (double)(v * v);
-    }
+    for (int P = 0; P < np; ++P) {
+        const uint4 w = *(const uint4 *)(p + (int64_t)P * 2048);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
-    if (lane == 0) out[row] = s;
+        for (int e = 0; e < 8; ++e) {
+            const float v =
+                (float)__builtin_bit_cast(__bf16, (uint16_t)(ws[e / 2] >> (16 * (e % 2))));
+            s += (double)(v * v);
+        }
+    }
+    s += __shfl_xor(s, 16, kWave);
+    s += __shfl_xor(s, 32, kWave);
+    if (lane < 16) out[grp * 16 + lane] = s;
 }
 
 }  // namespace
@@ -524,8 +531,8 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t st) {
         if (a.naux_pad % 32 != 0 || nrows % 16 != 0) return hipErrorInvalidValue;
         FH_LAUNCH(pack_bf16f_kernel, (char *)a.out0);
         if (a.lo_sq)
-            hipLaunchKernelGGL(lo_sq_frag_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0,
-                               st, (const char *)a.out0, nrows, a.naux_pad, a.lo_sq);
+            hipLaunchKernelGGL(lo_sq_frag_kernel, dim3((unsigned)((nrows / 16 + 3) / 4)), dim3(256),
+                               0, st, (const char *)a.out0, nrows, a.naux_pad, a.lo_sq);
         break;
     case PackFmt::FP8: {
         const int nqb = a.naux_pad / 64;
