@@ -91,7 +91,10 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(BatchArgs a) {
     __shared__ double tile[64][65];
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     const int l = threadIdx.x % 64, g = l / 16, rr = (threadIdx.x / 64) * 16 + l % 16;
-    for (int64_t t = blockIdx.x; t < a.nptiles; t += gridDim.x) {
+    // The next tile's 16 values per thread load while the current one is
+    // split and written.
+    double v[16];
+    auto fetch = [&](int64_t t) {
         const PackTile pt = a.ptiles[t];
         const DomDev dm = a.dom[pt.d];
         const int64_t nrows = (int64_t)dm.nocc * dm.nvir_pad;
@@ -103,14 +106,22 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(BatchArgs a) {
             const int aa = (int)(r % dm.nvir_pad);
             if (aa < dm.nvir) col = i * dm.nvir + aa;
         }
-        __syncthreads();
-        for (int q = ty; q < 64; q += 4) {
-            const int qq = pt.q0 + q;
-            double v = 0.0;
-            if (col >= 0 && qq < dm.naux) v = a.b[dm.b_off + (int64_t)qq * nov + col];
-            tile[q][tx] = v;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int qq = pt.q0 + ty + 4 * k;
+            v[k] = col >= 0 && qq < dm.naux ? a.b[dm.b_off + (int64_t)qq * nov + col] : 0.0;
         }
+    };
+    if ((int64_t)blockIdx.x < a.nptiles) fetch(blockIdx.x);
+    for (int64_t t = blockIdx.x; t < a.nptiles; t += gridDim.x) {
+        const PackTile pt = a.ptiles[t];
+        const DomDev dm = a.dom[pt.d];
+        const int64_t nrows = (int64_t)dm.nocc * dm.nvir_pad;
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) tile[ty + 4 * k][tx] = v[k];
+        __syncthreads();
+        if (t + gridDim.x < a.nptiles) fetch(t + gridDim.x);
         const int64_t orow = pt.r0 + rr;
         if (orow >= nrows) continue;
         const int np = dm.naux_pad / 32;
