@@ -62,7 +62,11 @@ __device__ __forceinline__ double combine_levels(int s2, int s3, int s4, int s5,
 // halves meet through LDS after the loop (an exact int32 merge of the level
 // sums for a single-product item; FP64 C2 for a two-product item), and waves
 // 0-3 run the energy epilogue from LDS-staged orbital energies and row
-// exponents. ADJ: phase A of the adjoint over every item at once, ending in
+// exponents. With a.q_rot_ticks > 0 an item starts at the Q panel the device
+// wall clock names (one panel per q_rot_ticks ticks, wrapping), so the items an
+// XCD runs at one moment read the same panel of the occupied rows they share
+// and the XCD's L2 holds it once; the int32 level sums are exact, so this
+// changes no bit. ADJ: phase A of the adjoint over every item at once, ending in
 // the T~ epilogue of pair_body.h (writing the whole-T~ matrix) instead of the
 // energy terms.
 template <int TILE, bool ADJ>
@@ -75,6 +79,7 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
     // and B and the row exponents of the four operand row sets.
     __shared__ double eps_t[2][TILE];
     __shared__ int rexp_t[4][TILE];
+    __shared__ int qrot;  // first Q panel of this item (one value for all waves)
 
     pair::Item it;
     it.id = a.item0 + pair_interleave((int64_t)blockIdx.x, a.ntp, a.nitems / a.ntp);
@@ -138,12 +143,20 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
 #pragma unroll
             for (int y = 0; y < NB; ++y) acc[l][x][y] = (v4i){0, 0, 0, 0};
 
+    // The waves of a single-product item split the panels by parity of their
+    // position after the rotation, so all of them must rotate by the same amount.
+    if (tid == 0)
+        qrot = a.q_rot_ticks > 0 ? (int)((wall_clock64() / (uint64_t)a.q_rot_ticks) % np) : 0;
+    __syncthreads();
+    const int rot = qrot;
+
     v4i fa[2][4][NB], fb[2][4][NB];
     // FULL: every block is loaded. Otherwise only the blocks inside nvir_pad,
     // the only ones the MFMAs below read (a 16-row block is used when both its
     // row block and its column block are inside).
     auto fetch = [&](int P, v4i(&ra)[4][NB], v4i(&rb)[4][NB], auto fullc) {
-        const int64_t off = (int64_t)P * FB;
+        const int R = P + rot;  // P < np and rot < np
+        const int64_t off = (int64_t)(R >= np ? R - np : R) * FB;
 #pragma unroll
         for (int d = 0; d < 4; ++d)
 #pragma unroll

@@ -292,6 +292,20 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t dep
 
 // ---- Energy core ---------------------------------------------------------------
 
+// Wall-clock ticks per Q panel for the INT8X4 panel rotation: 1.4 us, about
+// what one 64-deep panel of a 64-wide item takes when the chip is full
+// (FERRIC_I8_QROT_TICKS overrides; 0 switches the rotation off).
+int i8_q_rot_ticks(State &s) {
+    if (const char *e = getenv("FERRIC_I8_QROT_TICKS")) return std::max(0, atoi(e));
+    if (s.wall_khz < 0) {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s.device) != hipSuccess)
+            khz = 0;
+        s.wall_khz = std::max(0, khz);
+    }
+    return (int)std::lround(s.wall_khz * 1.4e-3);
+}
+
 fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux, size_t nocc,
                       size_t nvir, const double *eps, size_t first_occ, size_t nocc_total,
                       double kappa, Prec prec, hipStream_t stream, fh_spin_components *out,
@@ -416,6 +430,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     ea.diag_corr = pa.lo_sq;
     ea.rexp = pa.rexp;
     ea.opf = frag ? pa.out0 : nullptr;
+    ea.q_rot_ticks = prec == Prec::I8X4 ? i8_q_rot_ticks(s) : 0;
     ea.nocc = (int)nocc;
     ea.nvir = (int)nvir;
     ea.nvir_pad = pl.nvir_pad;

@@ -75,6 +75,7 @@ struct State {
     void *blas = nullptr;  // rocBLAS handle of the device, made on first use (blas_handle)
     // Phase timing (FERRIC_TIMING): events on the call's stream.
     int timing = -1;  // -1: not yet read from the environment
+    int wall_khz = -1;  // device wall-clock rate (kHz); -1: not yet queried
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     fh_timing last_timing{};
     // Fixed arena (fh_rimp2_energy_dev_noalloc): while set, reserve() carves
@@ -87,6 +88,8 @@ struct State {
 // Phase markers: record(k) for k = 0 (start), 1 (after pack), 2 (after the
 // contraction), 3 (after the reductions); finish() after the stream sync.
 bool timing_on(State &s);
+// Wall-clock ticks per Q panel for the INT8X4 kernels' panel rotation (0: off).
+int i8_q_rot_ticks(State &s);
 void mark(State &s, int k, hipStream_t st);
 void finish_timing(State &s, uint64_t items, int tile);
 

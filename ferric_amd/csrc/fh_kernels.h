@@ -132,6 +132,11 @@ struct EnergyArgs {
     const double *diag_corr;  // BF16: sum_Q lo^2 per packed row, added to K_ii[a, a]
     const int *rexp = nullptr;  // I8X4: per packed row, the exponent of its digit scale
     const void *opf = nullptr;  // BF16X3 / FP8, tile 128: the operand in MFMA fragment order
+    // I8X4: > 0 starts every item's Q panels at (device wall clock / q_rot_ticks)
+    // mod the panel count, so items running at the same moment read the same Q
+    // panel of their shared occupied rows; the int32 level sums are exact (modular),
+    // so the panel order changes no bit of the result. 0: panels from 0.
+    int q_rot_ticks = 0;
     int nocc, nvir, nvir_pad, naux_pad, nqb;
     const double *eps_occ;    // device, active occupied (nocc)
     const double *eps_vir;    // device, virtual (nvir)

@@ -70,6 +70,39 @@ def test_decane_scale_within_the_format_bound(oracle_mod):
     assert (s32.e_os, s32.e_ss) == (s64.e_os, s64.e_ss)
 
 
+def test_panel_rotation_changes_no_bit(monkeypatch):
+    """Each item may start its Q panels anywhere (a wall-clock phase): the int32
+    level sums are exact, so energies and the adjoint's Gamma / dE/deps are the
+    same bit for bit with the rotation off (0), changing every tick (1), at an
+    odd rate (7) and at the built-in rate. Shapes: an odd panel count with
+    single- and two-product items (41/209/960, 15 panels), the 32-wide tile, and
+    the hot-row distribution."""
+    cases = [synthetic.decane_scale(), synthetic.mixed_large(seed=11, nocc=12, nvir=20, naux=200),
+             synthetic.mixed_large(seed=12, nocc=9, nvir=130, naux=520)]
+    for p in cases:
+        t = torch.from_numpy(p.b_ov).cuda()
+        got = {}
+        for ticks in ("0", "1", "7", None):
+            if ticks is None:
+                monkeypatch.delenv("FERRIC_I8_QROT_TICKS", raising=False)
+            else:
+                monkeypatch.setenv("FERRIC_I8_QROT_TICKS", ticks)
+            sc = rimp2.spin_components(t, p.eps, p.nocc, p.nvir, precision=I8)
+            got[ticks] = (sc.e_os, sc.e_ss)
+        print("int8x4 rotation", p.nocc, p.nvir, p.b_ov.shape[0], got["0"], flush=True)
+        assert len(set(got.values())) == 1, got
+    p = synthetic.decane_scale()
+    t = torch.from_numpy(p.b_ov).cuda()
+    outs = []
+    for ticks in ("0", "1"):
+        monkeypatch.setenv("FERRIC_I8_QROT_TICKS", ticks)
+        sc, e, gam, deps = rimp2.energy_adjoint(t, p.eps, p.nocc, p.nvir, 0, p.nocc, 1.2, 1 / 3,
+                                                I8)
+        outs.append((e, gam.cpu().numpy(), np.asarray(deps)))
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+
+
 def test_mixed_workload_within_one_micro_hartree():
     p = synthetic.mixed_large()
     print("int8x4 mixed 100/900/3600", flush=True)
