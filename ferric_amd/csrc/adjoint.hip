@@ -884,7 +884,7 @@ static fh_status adjoint_core_i8(State &s, const void *b_dev, fh_dtype b_dtype, 
     ea.scale = nullptr;
     ea.diag_corr = nullptr;
     ea.rexp = pa.rexp;
-    ea.q_rot_ticks = i8_q_rot_ticks(s);
+    ea.q_rot_ticks = i8_q_rot_ticks(s, pl.naux_pad / 64, pl.ntp);
     ea.nocc = (int)nocc;
     ea.nvir = (int)nvir;
     ea.nvir_pad = pl.nvir_pad;

@@ -293,10 +293,14 @@ fh_status range_check(State &s, const void *b, bool in_f32, size_t n, size_t dep
 // ---- Energy core ---------------------------------------------------------------
 
 // Wall-clock ticks per Q panel for the INT8X4 panel rotation: 1.4 us, about
-// what one 64-deep panel of a 64-wide item takes when the chip is full
-// (FERRIC_I8_QROT_TICKS overrides; 0 switches the rotation off).
-int i8_q_rot_ticks(State &s) {
+// what one 64-deep panel of a 64-wide item takes when the chip is full. It pays
+// when an item's Q loop is long or one occupied pair has many tile pairs (at
+// least 40 panels or 32 tile pairs; measured +1 to +5 % there, -1 % below at
+// 41/209/960 and 100/200/2400), so smaller problems start at panel 0.
+// FERRIC_I8_QROT_TICKS overrides for every size (0 switches the rotation off).
+int i8_q_rot_ticks(State &s, int npanels, int ntp) {
     if (const char *e = getenv("FERRIC_I8_QROT_TICKS")) return std::max(0, atoi(e));
+    if (npanels < 40 && ntp < 32) return 0;
     if (s.wall_khz < 0) {
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s.device) != hipSuccess)
@@ -430,7 +434,7 @@ fh_status energy_core(State &s, const void *b_dev, fh_dtype b_dtype, size_t naux
     ea.diag_corr = pa.lo_sq;
     ea.rexp = pa.rexp;
     ea.opf = frag ? pa.out0 : nullptr;
-    ea.q_rot_ticks = prec == Prec::I8X4 ? i8_q_rot_ticks(s) : 0;
+    ea.q_rot_ticks = prec == Prec::I8X4 ? i8_q_rot_ticks(s, pl.naux_pad / 64, pl.ntp) : 0;
     ea.nocc = (int)nocc;
     ea.nvir = (int)nvir;
     ea.nvir_pad = pl.nvir_pad;

@@ -88,8 +88,9 @@ struct State {
 // Phase markers: record(k) for k = 0 (start), 1 (after pack), 2 (after the
 // contraction), 3 (after the reductions); finish() after the stream sync.
 bool timing_on(State &s);
-// Wall-clock ticks per Q panel for the INT8X4 kernels' panel rotation (0: off).
-int i8_q_rot_ticks(State &s);
+// Wall-clock ticks per Q panel for the INT8X4 kernels' panel rotation (0: off)
+// for a problem of npanels 64-deep Q panels and ntp tile pairs per occupied pair.
+int i8_q_rot_ticks(State &s, int npanels, int ntp);
 void mark(State &s, int k, hipStream_t st);
 void finish_timing(State &s, uint64_t items, int tile);
 
