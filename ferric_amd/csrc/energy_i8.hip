@@ -186,7 +186,11 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
     };
     // Iteration p covers panel p * pstep + poff (a single-product item's odd
     // panel past the end is skipped by its waves); unrolled by two so the
-    // register buffer of each step is static.
+    // register buffer of each step is static. The guarded fetches make the
+    // compiler wait for every outstanding load before a panel's MFMAs, the
+    // next panel's included, so the two waves of a SIMD take turns; fetching
+    // unconditionally (the next panel's loads in flight during the MFMAs) was
+    // measured 5 % slower at 100/900/3600.
     const int iters = sq.iters;
     auto valid = [&](int p) { return p < iters && p * pstep + poff < np; };
     auto run = [&](auto fullc) {
