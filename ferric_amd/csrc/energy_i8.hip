@@ -351,8 +351,8 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
 // (i, J, tA, tB) over every alpha i, beta J and tile of each side, one product
 // C = B_alpha(i, A)^T B_beta(J, B). Waves 0-3 take the even Q panels, waves 4-7
 // the odd ones; their int32 level sums merge exactly through LDS, and waves 0-3
-// add sum (ia|JB)^2 / D over the tile with weight 1. Operand format, fetch and
-// level combination as energy_i8x4_kernel; op0 / rexp hold the alpha rows
+// add sum (ia|JB)^2 / D over the tile with weight 1. Operand format, fetch,
+// wall-clock Q-panel rotation and level combination as energy_i8x4_kernel; op0 / rexp hold the alpha rows
 // (nvir_pad), op1 / rexp_b the beta rows (nvir_pad_b), each buffer with TILE
 // spare rows after its last.
 template <int TILE>
@@ -361,6 +361,7 @@ __global__ __launch_bounds__(512) void energy_i8x4_rect_kernel(EnergyArgs a) {
     constexpr int FB = 4096;
     __shared__ __attribute__((aligned(16))) int xbuf[kLevels * TILE * TILE];
     __shared__ int rexp_t[2][TILE];
+    __shared__ int qrot;  // first Q panel (the wall-clock rotation of energy_i8x4_kernel)
 
     pair::Item it;
     it.id = xcd_remap((int64_t)blockIdx.x, a.nitems);
@@ -410,9 +411,16 @@ __global__ __launch_bounds__(512) void energy_i8x4_rect_kernel(EnergyArgs a) {
 #pragma unroll
             for (int y = 0; y < NB; ++y) acc[l][x][y] = (v4i){0, 0, 0, 0};
 
+    // The even / odd panel split of the two halves needs one rotation for all waves.
+    if (tid == 0)
+        qrot = a.q_rot_ticks > 0 ? (int)((wall_clock64() / (uint64_t)a.q_rot_ticks) % np) : 0;
+    __syncthreads();
+    const int rot = qrot;
+
     v4i fa[2][4][NB], fb[2][4][NB];
     auto fetch = [&](int P, v4i(&ra)[4][NB], v4i(&rb)[4][NB]) {
-        const int64_t off = (int64_t)P * FB;
+        const int R = P + rot;  // P < np and rot < np
+        const int64_t off = (int64_t)(R >= np ? R - np : R) * FB;
 #pragma unroll
         for (int d = 0; d < 4; ++d)
 #pragma unroll

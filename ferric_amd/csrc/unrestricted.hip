@@ -153,6 +153,7 @@ fh_status opposite_core(State &s, const fh_spin_channel &ca, const fh_spin_chann
     ea.eps_vir_b = deps + ca.nocc + ca.nvir + cb.nocc;
     ea.rexp = rexp_a;
     ea.rexp_b = rexp_b;
+    ea.q_rot_ticks = i8 ? i8_q_rot_ticks(s, ea.naux_pad / 64, ea.ntile * ea.ntile_b) : 0;
     mark(s, 1, stream);
     he = launch_energy_rect(ea, stream);
     if (he != hipSuccess) {

@@ -91,6 +91,17 @@ def test_panel_rotation_changes_no_bit(monkeypatch):
             got[ticks] = (sc.e_os, sc.e_ss)
         print("int8x4 rotation", p.nocc, p.nvir, p.b_ov.shape[0], got["0"], flush=True)
         assert len(set(got.values())) == 1, got
+    # the opposite-spin rectangle (alpha and beta halves split the panels by parity)
+    from ferric_amd import unrestricted as un
+    pa = synthetic.mixed_large(seed=13, nocc=9, nvir=70, naux=520)
+    pb = synthetic.mixed_large(seed=14, nocc=8, nvir=75, naux=520)
+    ca = un.SpinChannel(torch.from_numpy(pa.b_ov).cuda(), pa.eps, pa.nocc, pa.nvir)
+    cb = un.SpinChannel(torch.from_numpy(pb.b_ov).cuda(), pb.eps, pb.nocc, pb.nvir)
+    got = set()
+    for ticks in ("0", "1", "7"):
+        monkeypatch.setenv("FERRIC_I8_QROT_TICKS", ticks)
+        got.add(un.u_opposite_spin(ca, cb, I8))
+    assert len(got) == 1, got
     p = synthetic.decane_scale()
     t = torch.from_numpy(p.b_ov).cuda()
     outs = []
