@@ -352,9 +352,9 @@ __global__ __launch_bounds__(512) void energy_i8x4_kernel(EnergyArgs a) {
 // C = B_alpha(i, A)^T B_beta(J, B). Waves 0-3 take the even Q panels, waves 4-7
 // the odd ones; their int32 level sums merge exactly through LDS, and waves 0-3
 // add sum (ia|JB)^2 / D over the tile with weight 1. Operand format, fetch,
-// wall-clock Q-panel rotation and level combination as energy_i8x4_kernel; op0 / rexp hold the alpha rows
-// (nvir_pad), op1 / rexp_b the beta rows (nvir_pad_b), each buffer with TILE
-// spare rows after its last.
+// wall-clock Q-panel rotation and level combination as energy_i8x4_kernel;
+// op0 / rexp hold the alpha rows (nvir_pad), op1 / rexp_b the beta rows
+// (nvir_pad_b), each buffer with TILE spare rows after its last.
 template <int TILE>
 __global__ __launch_bounds__(512) void energy_i8x4_rect_kernel(EnergyArgs a) {
     constexpr int TW = TILE / 2, NB = TW / 16;
